@@ -1,0 +1,116 @@
+/*
+ * pele_chem_hip: C-ABI of the MI355X (gfx950) chemistry integrators.
+ *
+ * The entry points replace the GPU work of the reference reactors behind
+ * pele::physics::reactions::ReactorBase (Source/Reactions/ReactorBase.H:14-106):
+ *
+ *   pp_chem_create / pp_chem_config_init
+ *       <- ReactorBase::create(name) + ReactorX::init(reactor_type, ncells)
+ *          (Source/Factory.H:72-79, ReactorBDF.cpp:112-130, ReactorRK64.cpp:6-20);
+ *          the ode.* ParmParse keys become config fields.
+ *   pp_chem_react_box
+ *       <- ReactorX::react(const Box&, Array4 rY, rYsrc, T, rE, rEsrc, FC, mask,
+ *          Real& dt_react, Real& time, gpuStream_t)
+ *          (ReactorBase.H:25-40; ReactorBDF.cpp:314-494; ReactorRK64.cpp:166-320).
+ *   pp_chem_react_1d
+ *       <- ReactorX::react(Real* rY, Real* rYsrc, Real* rX, Real* rXsrc,
+ *          Real& dt_react, Real& time, int ncells, gpuStream_t)
+ *          (ReactorBase.H:42-54; ReactorBDF.cpp:133-311; ReactorRK64.cpp:22-164).
+ *   pp_chem_destroy
+ *       <- ReactorX::close() + delete (ReactorBase.H:21).
+ *
+ * Plain C types only: pointers, sizes, an Array4-shaped view struct.
+ * Return values: >= 0 is int(average steps per cell) as the reference's
+ * react() returns; < 0 is an error code (never abort); pp_chem_last_error()
+ * gives the message.  A handle may be used from one host thread at a time.
+ */
+#ifndef PELE_CHEM_HIP_H
+#define PELE_CHEM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* integrator selection (string names of ReactorBase::create) */
+enum {
+  PP_CHEM_BDF = 0,                 /* "ReactorBDF": fixed-step BDF, GMRES (parity) */
+  PP_CHEM_RK64 = 1,                /* "ReactorRK64": explicit low-storage RK64 (parity) */
+  PP_CHEM_BDF_ADAPTIVE = 2,        /* adaptive-order BDF with error control, FP64 LU */
+  PP_CHEM_BDF_ADAPTIVE_MIXED = 3   /* adaptive BDF, FP32 Newton-matrix LU, FP64 elsewhere */
+};
+
+enum {
+  PP_CHEM_OK = 0,
+  PP_CHEM_EINVAL = -1,    /* bad argument or config (e.g. reactor_type not 1 or 2) */
+  PP_CHEM_EMECH = -2,     /* unknown mechanism name */
+  PP_CHEM_EHIP = -3,      /* HIP runtime error */
+  PP_CHEM_EINTEGRATE = -4 /* a cell failed (NaN in GMRES, step limit, step too small) */
+};
+
+typedef struct {
+  const char* mechanism; /* "LiDryer", "drm19", ... */
+  int reactor_type;      /* 1: e (constant volume, rho e), 2: h (rho h) */
+  int mode;              /* PP_CHEM_* integrator */
+  int mod_reactor;       /* 1: time += dt_react on return (-DMOD_REACTOR) */
+  /* ReactorBDF ode.bdf_* keys (ReactorBDF.H:118-128) */
+  int bdf_nsubsteps, bdf_nonlinear_iters, bdf_gmres_restarts, bdf_gmres_kspiters;
+  int bdf_gmres_precond, bdf_scheme;
+  double bdf_nonlinear_tol, bdf_gmres_tol;
+  /* ReactorRK64 ode.* keys (ReactorRK64.H:131-134) */
+  double rk64_atol;
+  int rk64_nsubsteps_guess, rk64_nsubsteps_min, rk64_nsubsteps_max;
+  /* adaptive BDF */
+  double rtol, atol, first_step;
+  int max_steps;
+  int block_size; /* threads per block, 0: default (64) */
+} pp_chem_config;
+
+/* amrex::Array4<Real> layout: element (i,j,k,n) is at
+ * p[(i-begin[0]) + (j-begin[1])*jstride + (k-begin[2])*kstride + n*nstride]. */
+typedef struct {
+  double* p;
+  long long jstride, kstride, nstride;
+  int begin[3];
+  int end[3];
+  int ncomp;
+} pp_array4;
+
+typedef struct {
+  long long ncells, nsteps, nrhs, njac, nlu, nnewton, nreject;
+  int max_steps_cell, min_steps_cell, nfailed;
+  double kernel_ms; /* device time of the last integration kernel */
+} pp_chem_stats;
+
+typedef struct pp_chem pp_chem;
+
+/* Fill cfg with the reference defaults for the given mechanism and mode. */
+int pp_chem_config_init(pp_chem_config* cfg, const char* mechanism, int mode, int reactor_type);
+int pp_chem_create(const pp_chem_config* cfg, pp_chem** out);
+void pp_chem_destroy(pp_chem* h);
+int pp_chem_num_species(const pp_chem* h);
+/* Comma-separated list of compiled mechanisms; returns the needed length. */
+int pp_chem_list_mechanisms(char* buf, int buflen);
+
+/* Box form: device-resident Array4 views over [lo, hi] (inclusive), updated
+ * in place (rY, T, rE, FC); rYsrc, rEsrc read; mask ignored as in the
+ * reference (ReactorBDF.cpp:322).  FC may have p == NULL. */
+int pp_chem_react_box(pp_chem* h, const int lo[3], const int hi[3], const pp_array4* rY,
+                      const pp_array4* rYsrc, const pp_array4* T, const pp_array4* rE,
+                      const pp_array4* rEsrc, const pp_array4* FC, const pp_array4* mask,
+                      double* dt_react, double* time, void* stream);
+
+/* 1-D form: y[icell*(N+1)+k] (k = N is T), ysrc[icell*N+k], rX/rXsrc[icell];
+ * on_device = 0: host pointers (copied in and out, as the reference does on
+ * GPU builds, ReactorBDF.cpp:155-171,295-302); 1: device pointers. */
+int pp_chem_react_1d(pp_chem* h, double* y, double* ysrc, double* rX, double* rXsrc,
+                     double* dt_react, double* time, int ncells, int on_device, void* stream);
+
+int pp_chem_get_stats(const pp_chem* h, pp_chem_stats* out);
+/* Per-cell accepted step counts of the last call (host copy). */
+int pp_chem_copy_cell_steps(pp_chem* h, int* host_out, int ncells);
+const char* pp_chem_last_error(const pp_chem* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PELE_CHEM_HIP_H */

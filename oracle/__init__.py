@@ -1,0 +1,312 @@
+"""ctypes wrapper of the CPU oracle (``ppc_oracle.c``).
+
+TEST INFRASTRUCTURE: imported only by ``tests/``, ``__graft_entry__.smoke``
+and the ``cpu_baseline`` leg of ``bench.py``, as the checker.  The product
+(``pelephysics_amd``) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libppc_oracle.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class BDFParams(C.Structure):
+    _fields_ = [("nsubsteps", C.c_int), ("nonlinear_iters", C.c_int),
+                ("gmres_restarts", C.c_int), ("gmres_kspiters", C.c_int),
+                ("gmres_precond", C.c_int), ("scheme", C.c_int),
+                ("nonlin_tol", C.c_double), ("gmres_tol", C.c_double)]
+
+
+class RK64Params(C.Structure):
+    _fields_ = [("atol", C.c_double), ("nsub_guess", C.c_int),
+                ("nsub_min", C.c_int), ("nsub_max", C.c_int)]
+
+
+class AdaptParams(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double),
+                ("first_step", C.c_double), ("max_steps", C.c_int)]
+
+
+class AdaptStats(C.Structure):
+    _fields_ = [("nsteps", C.c_int), ("nrhs", C.c_int), ("njac", C.c_int),
+                ("nlu", C.c_int), ("nnewton", C.c_int), ("nreject", C.c_int),
+                ("nfail", C.c_int), ("status", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with make (gcc); returns the library path."""
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < max(
+            os.path.getmtime(os.path.join(HERE, f)) for f in ("ppc_oracle.c", "ppc_oracle.h")):
+        subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(LIB)
+        vp = C.c_void_p
+        L.ppo_mech_create.restype = vp
+        L.ppo_mech_create.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip,
+                                      _dp, _dp, _ip, _dp, _dp, _dp, _dp]
+        L.ppo_mech_destroy.argtypes = [vp]
+        L.ppo_mech_set_dlnA.argtypes = [vp, C.c_void_p]
+        for nm in ("ppo_cp_R", "ppo_h_RT", "ppo_g_RT"):
+            getattr(L, nm).argtypes = [vp, C.c_double, _dp]
+        for nm in ("ppo_hbms_cpbs", "ppo_ubms_cvbs"):
+            getattr(L, nm).argtypes = [vp, C.c_double, _dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        for nm in ("ppo_T_given_hy", "ppo_T_given_ey"):
+            getattr(L, nm).argtypes = [vp, C.c_double, _dp, C.c_double, C.POINTER(C.c_int)]
+            getattr(L, nm).restype = C.c_double
+        L.ppo_production_rate.argtypes = [vp, _dp, C.c_double, _dp]
+        L.ppo_progress_rates.argtypes = [vp, _dp, C.c_double, _dp]
+        L.ppo_jacobian.argtypes = [vp, _dp, C.c_double, C.c_int, _dp]
+        L.ppo_rty2wdot.argtypes = [vp, C.c_double, C.c_double, _dp, _dp]
+        L.ppo_dwdot.argtypes = [vp, _dp, C.c_double, C.c_int, _dp]
+        L.ppo_fkernelspec.argtypes = [vp, C.c_double, C.c_int, _dp, _dp, C.c_double, C.c_double, _dp]
+        L.ppo_chem_jac.argtypes = [vp, C.c_int, _dp, _dp]
+        L.ppo_react_bdf_1d.argtypes = [vp, C.POINTER(BDFParams), C.c_int, _dp, _dp, _dp, _dp,
+                                       C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_int]
+        L.ppo_react_rk64_1d.argtypes = [vp, C.POINTER(RK64Params), C.c_int, _dp, _dp, _dp, _dp,
+                                        C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_int]
+        L.ppo_react_adaptive_1d.argtypes = [vp, C.POINTER(AdaptParams), C.c_int, _dp, _dp, _dp,
+                                            _dp, C.c_double, C.c_double, C.c_int,
+                                            C.POINTER(AdaptStats), C.c_int]
+        L.ppo_react_box.argtypes = [vp, C.c_int, C.POINTER(BDFParams), C.POINTER(RK64Params),
+                                    C.POINTER(AdaptParams), C.c_int, _dp, _dp, _dp, _dp, _dp,
+                                    _dp, C.c_double, C.c_double, C.c_int,
+                                    C.POINTER(AdaptStats), C.c_int]
+        L.ppo_bdf_linear.argtypes = [C.c_int, _dp, C.c_void_p, _dp, C.c_double, C.POINTER(BDFParams)]
+        L.ppo_rk64_linear.argtypes = [C.c_int, _dp, C.c_void_p, _dp, C.c_double, C.POINTER(RK64Params)]
+        L.ppo_adaptive_linear.argtypes = [C.c_int, _dp, C.c_void_p, _dp, C.c_double,
+                                          C.POINTER(AdaptParams), C.POINTER(AdaptStats)]
+        _lib = L
+    return _lib
+
+
+def bdf_params(ns: int, **kw) -> BDFParams:
+    """ReactorBDF defaults (Source/Reactions/ReactorBDF.H:118-128)."""
+    d = dict(nsubsteps=1, nonlinear_iters=1, gmres_restarts=10, gmres_kspiters=ns + 1,
+             gmres_precond=2, scheme=3, nonlin_tol=1e-6, gmres_tol=1e-10)
+    d.update(kw)
+    return BDFParams(**d)
+
+
+def rk64_params(**kw) -> RK64Params:
+    """ReactorRK64 defaults (Source/Reactions/ReactorRK64.H:131-134)."""
+    d = dict(atol=1e-10, nsub_guess=10, nsub_min=5, nsub_max=500)
+    d.update(kw)
+    return RK64Params(**d)
+
+
+def adapt_params(rtol=1e-8, atol=1e-10, first_step=0.0, max_steps=100000) -> AdaptParams:
+    return AdaptParams(rtol, atol, first_step, max_steps)
+
+
+class Mech:
+    """Oracle view of one mechanism (tables from pelephysics_amd/mechanisms/<name>.json)."""
+
+    def __init__(self, model):
+        self.model = model
+        ns, nr = model.n_species, model.n_reactions
+        self.ns, self.nr, self.n = ns, nr, ns + 1
+        sp = model.species
+        self.mw = np.array([s.mw for s in sp])
+        self.imw = np.array([s.imw for s in sp])
+        tmid = np.array([s.tmid for s in sp])
+        lo = np.array([s.low for s in sp]).reshape(-1)
+        hi = np.array([s.high for s in sp]).reshape(-1)
+        kind = np.zeros(nr, np.int32)
+        rev = np.zeros(nr, np.int32)
+        arr = np.zeros((nr, 3))
+        arrl = np.zeros((nr, 3))
+        troen = np.zeros(nr, np.int32)
+        troe = np.zeros((nr, 4))
+        eff = np.zeros((nr, ns))
+        nur = np.zeros((nr, ns))
+        nup = np.zeros((nr, ns))
+        for j, r in enumerate(model.reactions):
+            kind[j] = {"elementary": 0, "three-body": 1, "falloff": 2}[r.kind]
+            rev[j] = int(r.reversible)
+            arr[j] = [r.rate.A, r.rate.b, r.rate.EaR]
+            if r.low is not None:
+                arrl[j] = [r.low.A, r.low.b, r.low.EaR]
+            if r.troe is not None:
+                troen[j] = len(r.troe)
+                troe[j, :len(r.troe)] = r.troe
+            if r.has_third_body:
+                eff[j, :] = r.default_eff
+                for k, e in r.efficiencies.items():
+                    eff[j, k] = e
+            for k, v in r.reactants.items():
+                nur[j, k] = v
+            for k, v in r.products.items():
+                nup[j, k] = v
+        self._keep = [tmid, lo, hi, kind, rev, arr, arrl, troen, troe, eff, nur, nup]
+        self.nu_net = nup - nur
+        self.h = lib().ppo_mech_create(ns, nr, self.mw, self.imw, tmid, lo, hi, kind, rev,
+                                        arr.reshape(-1), arrl.reshape(-1), troen,
+                                        troe.reshape(-1), eff.reshape(-1), nur.reshape(-1),
+                                        nup.reshape(-1))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ppo_mech_destroy(self.h)
+            self.h = None
+
+    @staticmethod
+    def load(name: str) -> "Mech":
+        from pelephysics_amd.mechgen.model import Mechanism
+        path = os.path.join(os.path.dirname(HERE), "pelephysics_amd", "mechanisms", f"{name}.json")
+        with open(path) as f:
+            return Mech(Mechanism.from_json(f.read()))
+
+    # -- thermo
+    def cp_R(self, T):
+        o = np.zeros(self.ns); lib().ppo_cp_R(self.h, T, o); return o
+
+    def h_RT(self, T):
+        o = np.zeros(self.ns); lib().ppo_h_RT(self.h, T, o); return o
+
+    def g_RT(self, T):
+        o = np.zeros(self.ns); lib().ppo_g_RT(self.h, T, o); return o
+
+    def h_cp_mass(self, T, Y):
+        h, c = C.c_double(), C.c_double()
+        lib().ppo_hbms_cpbs(self.h, T, np.ascontiguousarray(Y, float), C.byref(h), C.byref(c))
+        return h.value, c.value
+
+    def e_cv_mass(self, T, Y):
+        h, c = C.c_double(), C.c_double()
+        lib().ppo_ubms_cvbs(self.h, T, np.ascontiguousarray(Y, float), C.byref(h), C.byref(c))
+        return h.value, c.value
+
+    def T_from_h(self, h, Y, Tguess):
+        e = C.c_int()
+        T = lib().ppo_T_given_hy(self.h, h, np.ascontiguousarray(Y, float), Tguess, C.byref(e))
+        return T, e.value
+
+    def T_from_e(self, ev, Y, Tguess):
+        e = C.c_int()
+        T = lib().ppo_T_given_ey(self.h, ev, np.ascontiguousarray(Y, float), Tguess, C.byref(e))
+        return T, e.value
+
+    # -- kinetics (SI-mol)
+    def wdot(self, sc, T):
+        o = np.zeros(self.ns)
+        lib().ppo_production_rate(self.h, np.ascontiguousarray(sc, float), T, o)
+        return o
+
+    def qdot(self, sc, T):
+        o = np.zeros(self.nr)
+        lib().ppo_progress_rates(self.h, np.ascontiguousarray(sc, float), T, o)
+        return o
+
+    def jacobian(self, sc, T, consP):
+        o = np.zeros(self.n * self.n)
+        lib().ppo_jacobian(self.h, np.ascontiguousarray(sc, float), T, consP, o)
+        return o
+
+    def set_dlnA(self, dlnA=None):
+        if dlnA is None:
+            lib().ppo_mech_set_dlnA(self.h, None)
+        else:
+            self._dlnA = np.ascontiguousarray(dlnA, float)
+            lib().ppo_mech_set_dlnA(self.h, self._dlnA.ctypes.data)
+
+    # -- reactor
+    def fkernelspec(self, y, rhoe_init, rhoesrc, rYsrc, reactor_type=2, dt_save=0.0):
+        o = np.zeros(self.n)
+        lib().ppo_fkernelspec(self.h, dt_save, reactor_type, np.ascontiguousarray(y, float), o,
+                              rhoe_init, rhoesrc, np.ascontiguousarray(rYsrc, float))
+        return o
+
+    def chem_jac(self, y, reactor_type=2):
+        o = np.zeros(self.n * self.n)
+        lib().ppo_chem_jac(self.h, reactor_type, np.ascontiguousarray(y, float), o)
+        return o.reshape(self.n, self.n)
+
+    def react_1d(self, mode, y, ysrc, rX, rXsrc, dt, reactor_type=2, params=None, nthreads=0,
+                 time=0.0):
+        """mode: 'bdf' | 'rk64' | 'adaptive'.  Arrays are updated in place."""
+        ncells = rX.shape[0]
+        if mode == "bdf":
+            p = params or bdf_params(self.ns)
+            fc = np.zeros(ncells, np.int32)
+            r = lib().ppo_react_bdf_1d(self.h, C.byref(p), reactor_type, y, ysrc, rX, rXsrc, dt,
+                                       time, ncells, fc.ctypes.data, nthreads)
+            return r, fc
+        if mode == "rk64":
+            p = params or rk64_params()
+            fc = np.zeros(ncells, np.int32)
+            r = lib().ppo_react_rk64_1d(self.h, C.byref(p), reactor_type, y, ysrc, rX, rXsrc, dt,
+                                        time, ncells, fc.ctypes.data, nthreads)
+            return r, fc
+        p = params or adapt_params()
+        st = AdaptStats()
+        r = lib().ppo_react_adaptive_1d(self.h, C.byref(p), reactor_type, y, ysrc, rX, rXsrc, dt,
+                                        time, ncells, C.byref(st), nthreads)
+        return r, st.as_dict()
+
+    def react_box(self, mode, rY, rYsrc, T, rE, rEsrc, dt, reactor_type=2, params=None,
+                  nthreads=0, time=0.0):
+        """Contiguous SoA box (rY[k, cell]); mode 'bdf' | 'rk64' | 'adaptive'."""
+        ncells = T.shape[0]
+        FC = np.zeros(ncells)
+        pb = params if mode == "bdf" and params is not None else bdf_params(self.ns)
+        pr = params if mode == "rk64" and params is not None else rk64_params()
+        pa = params if mode == "adaptive" and params is not None else adapt_params()
+        st = AdaptStats()
+        md = {"bdf": 0, "rk64": 1, "adaptive": 2}[mode]
+        r = lib().ppo_react_box(self.h, md, C.byref(pb), C.byref(pr), C.byref(pa), reactor_type,
+                                rY.reshape(-1), rYsrc.reshape(-1), T, rE, rEsrc, FC, dt, time,
+                                ncells, C.byref(st), nthreads)
+        return r, FC
+
+
+def bdf_linear(A, y, dt, b=None, **kw):
+    n = A.shape[0]
+    y = np.ascontiguousarray(y, float).copy()
+    p = bdf_params(n - 1, **kw)
+    bb = None if b is None else np.ascontiguousarray(b, float)
+    r = lib().ppo_bdf_linear(n, np.ascontiguousarray(A, float).reshape(-1),
+                             None if bb is None else bb.ctypes.data, y, dt, C.byref(p))
+    return r, y
+
+
+def rk64_linear(A, y, dt, b=None, **kw):
+    n = A.shape[0]
+    y = np.ascontiguousarray(y, float).copy()
+    p = rk64_params(**kw)
+    bb = None if b is None else np.ascontiguousarray(b, float)
+    r = lib().ppo_rk64_linear(n, np.ascontiguousarray(A, float).reshape(-1),
+                              None if bb is None else bb.ctypes.data, y, dt, C.byref(p))
+    return r, y
+
+
+def adaptive_linear(A, y, t_end, b=None, **kw):
+    n = A.shape[0]
+    y = np.ascontiguousarray(y, float).copy()
+    p = adapt_params(**kw)
+    st = AdaptStats()
+    bb = None if b is None else np.ascontiguousarray(b, float)
+    r = lib().ppo_adaptive_linear(n, np.ascontiguousarray(A, float).reshape(-1),
+                                  None if bb is None else bb.ctypes.data, y, t_end, C.byref(p),
+                                  C.byref(st))
+    return r, y, st.as_dict()

@@ -1,0 +1,1271 @@
+/*
+ * CPU oracle of the PelePhysics Reactions hot path -- TEST INFRASTRUCTURE.
+ * See ppc_oracle.h.  Plain C99 + OpenMP, table-driven (loops over reactions),
+ * independent of the generated HIP device code.
+ */
+#include "ppc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define RU 8.31446261815324e7 /* Source/PhysicsConstants.H:11 */
+#define LN10 2.302585092994045684
+#define LOG10E 0.4342944819032518277
+
+struct ppo_mech {
+  int ns, nr;
+  double *mw, *imw, *tmid, *lo, *hi; /* lo/hi: ns x 7 */
+  int *kind, *rev, *troe_n;
+  double *arr, *arr_low, *troe, *eff, *nur, *nup, *scale;
+  /* sparse lists per reaction */
+  int *nr_r, *nr_p, *nr_n; /* counts */
+  int *ir, *ip, *in;       /* species indices (max 8 per reaction) */
+  double *vr, *vp, *vn;    /* coefficients */
+  double *delta;           /* sum nu' - sum nu'' */
+};
+
+#define MAXSP 8
+
+ppo_mech* ppo_mech_create(int ns, int nr, const double* mw, const double* imw,
+                          const double* tmid, const double* nasa_lo,
+                          const double* nasa_hi, const int* kind,
+                          const int* reversible, const double* arr,
+                          const double* arr_low, const int* troe_n,
+                          const double* troe, const double* eff,
+                          const double* nu_reac, const double* nu_prod) {
+  ppo_mech* m = (ppo_mech*)calloc(1, sizeof(ppo_mech));
+  m->ns = ns;
+  m->nr = nr;
+#define DUP(dst, src, cnt, T)                       \
+  do {                                              \
+    m->dst = (T*)malloc(sizeof(T) * (size_t)(cnt)); \
+    memcpy(m->dst, src, sizeof(T) * (size_t)(cnt)); \
+  } while (0)
+  DUP(mw, mw, ns, double);
+  DUP(imw, imw, ns, double);
+  DUP(tmid, tmid, ns, double);
+  DUP(lo, nasa_lo, ns * 7, double);
+  DUP(hi, nasa_hi, ns * 7, double);
+  DUP(kind, kind, nr, int);
+  DUP(rev, reversible, nr, int);
+  DUP(troe_n, troe_n, nr, int);
+  DUP(arr, arr, nr * 3, double);
+  DUP(arr_low, arr_low, nr * 3, double);
+  DUP(troe, troe, nr * 4, double);
+  DUP(eff, eff, (size_t)nr * ns, double);
+  DUP(nur, nu_reac, (size_t)nr * ns, double);
+  DUP(nup, nu_prod, (size_t)nr * ns, double);
+#undef DUP
+  m->scale = (double*)malloc(sizeof(double) * nr);
+  for (int j = 0; j < nr; ++j) m->scale[j] = 1.0;
+  m->nr_r = (int*)calloc(nr, sizeof(int));
+  m->nr_p = (int*)calloc(nr, sizeof(int));
+  m->nr_n = (int*)calloc(nr, sizeof(int));
+  m->ir = (int*)calloc((size_t)nr * MAXSP, sizeof(int));
+  m->ip = (int*)calloc((size_t)nr * MAXSP, sizeof(int));
+  m->in = (int*)calloc((size_t)nr * MAXSP, sizeof(int));
+  m->vr = (double*)calloc((size_t)nr * MAXSP, sizeof(double));
+  m->vp = (double*)calloc((size_t)nr * MAXSP, sizeof(double));
+  m->vn = (double*)calloc((size_t)nr * MAXSP, sizeof(double));
+  m->delta = (double*)calloc(nr, sizeof(double));
+  for (int j = 0; j < nr; ++j) {
+    double d = 0.0;
+    for (int k = 0; k < ns; ++k) {
+      double a = m->nur[(size_t)j * ns + k], b = m->nup[(size_t)j * ns + k];
+      d += a - b;
+      if (a != 0.0) {
+        m->ir[j * MAXSP + m->nr_r[j]] = k;
+        m->vr[j * MAXSP + m->nr_r[j]++] = a;
+      }
+      if (b != 0.0) {
+        m->ip[j * MAXSP + m->nr_p[j]] = k;
+        m->vp[j * MAXSP + m->nr_p[j]++] = b;
+      }
+      if (b - a != 0.0) {
+        m->in[j * MAXSP + m->nr_n[j]] = k;
+        m->vn[j * MAXSP + m->nr_n[j]++] = b - a;
+      }
+    }
+    m->delta[j] = d;
+  }
+  return m;
+}
+
+void ppo_mech_destroy(ppo_mech* m) {
+  if (!m) return;
+  free(m->mw); free(m->imw); free(m->tmid); free(m->lo); free(m->hi);
+  free(m->kind); free(m->rev); free(m->troe_n); free(m->arr); free(m->arr_low);
+  free(m->troe); free(m->eff); free(m->nur); free(m->nup); free(m->scale);
+  free(m->nr_r); free(m->nr_p); free(m->nr_n); free(m->ir); free(m->ip);
+  free(m->in); free(m->vr); free(m->vp); free(m->vn); free(m->delta);
+  free(m);
+}
+
+void ppo_mech_set_dlnA(ppo_mech* m, const double* dlnA) {
+  for (int j = 0; j < m->nr; ++j) m->scale[j] = dlnA ? exp(dlnA[j]) : 1.0;
+}
+
+/* ------------------------------------------------------------- thermo -- */
+static const double* nasa(const ppo_mech* m, int k, double T) {
+  return (T < m->tmid[k]) ? m->lo + 7 * k : m->hi + 7 * k;
+}
+
+void ppo_cp_R(const ppo_mech* m, double T, double* out) {
+  for (int k = 0; k < m->ns; ++k) {
+    const double* a = nasa(m, k, T);
+    out[k] = a[0] + a[1] * T + a[2] * T * T + a[3] * T * T * T + a[4] * T * T * T * T;
+  }
+}
+
+void ppo_h_RT(const ppo_mech* m, double T, double* out) {
+  for (int k = 0; k < m->ns; ++k) {
+    const double* a = nasa(m, k, T);
+    out[k] = a[0] + a[1] / 2.0 * T + a[2] / 3.0 * T * T + a[3] / 4.0 * T * T * T +
+             a[4] / 5.0 * T * T * T * T + a[5] / T;
+  }
+}
+
+static void s_R(const ppo_mech* m, double T, double* out) {
+  for (int k = 0; k < m->ns; ++k) {
+    const double* a = nasa(m, k, T);
+    out[k] = a[0] * log(T) + a[1] * T + a[2] / 2.0 * T * T + a[3] / 3.0 * T * T * T +
+             a[4] / 4.0 * T * T * T * T + a[6];
+  }
+}
+
+/* g/RT = h/RT - s/R (mechanism.H gibbs :561) */
+void ppo_g_RT(const ppo_mech* m, double T, double* out) {
+  double s[128];
+  double* sb = m->ns > 128 ? (double*)malloc(sizeof(double) * m->ns) : s;
+  ppo_h_RT(m, T, out);
+  s_R(m, T, sb);
+  for (int k = 0; k < m->ns; ++k) out[k] -= sb[k];
+  if (sb != s) free(sb);
+}
+
+static void dcpR_dT(const ppo_mech* m, double T, double* out) {
+  for (int k = 0; k < m->ns; ++k) {
+    const double* a = nasa(m, k, T);
+    out[k] = a[1] + 2.0 * a[2] * T + 3.0 * a[3] * T * T + 4.0 * a[4] * T * T * T;
+  }
+}
+
+/* CKHBMS / CKCPBS (mechanism.H:2170,1692): sum_k Y_k/W_k * (h/RT) * RT, cp */
+void ppo_hbms_cpbs(const ppo_mech* m, double T, const double* Y, double* h, double* cp) {
+  double hs = 0.0, cs = 0.0;
+  for (int k = 0; k < m->ns; ++k) {
+    const double* a = nasa(m, k, T);
+    double cpr = a[0] + a[1] * T + a[2] * T * T + a[3] * T * T * T + a[4] * T * T * T * T;
+    double hrt = a[0] + a[1] / 2.0 * T + a[2] / 3.0 * T * T + a[3] / 4.0 * T * T * T +
+                 a[4] / 5.0 * T * T * T * T + a[5] / T;
+    hs += Y[k] * m->imw[k] * hrt;
+    cs += Y[k] * m->imw[k] * cpr;
+  }
+  *h = hs * RU * T;
+  *cp = cs * RU;
+}
+
+/* CKUBMS / CKCVBS (mechanism.H:2452,1931): e = h - RT, cv = cp - R per mole */
+void ppo_ubms_cvbs(const ppo_mech* m, double T, const double* Y, double* e, double* cv) {
+  double es = 0.0, cs = 0.0;
+  for (int k = 0; k < m->ns; ++k) {
+    const double* a = nasa(m, k, T);
+    double cpr = a[0] + a[1] * T + a[2] * T * T + a[3] * T * T * T + a[4] * T * T * T * T;
+    double hrt = a[0] + a[1] / 2.0 * T + a[2] / 3.0 * T * T + a[3] / 4.0 * T * T * T +
+                 a[4] / 5.0 * T * T * T * T + a[5] / T;
+    es += Y[k] * m->imw[k] * (hrt - 1.0);
+    cs += Y[k] * m->imw[k] * (cpr - 1.0);
+  }
+  *e = es * RU * T;
+  *cv = cs * RU;
+}
+
+/* GET_T_GIVEN_HY / _EY (mechanism.H:2789-2890): Newton on T with |dT| clamped to
+ * 100 K, stop when |dT| < 1e-6 (without applying it), 200 iterations, linear
+ * extrapolation outside [90, 4000] K. */
+static double T_given(const ppo_mech* m, int use_h, double hv, const double* Y,
+                      double t, int* ierr) {
+  const int maxiter = 200;
+  const double tol = 1.e-6;
+  const double tmin = 90, tmax = 4000;
+  double hmin, hmax, c, h1, t1, dt;
+  if (use_h) {
+    ppo_hbms_cpbs(m, tmin, Y, &hmin, &c);
+    ppo_hbms_cpbs(m, tmax, Y, &hmax, &c);
+  } else {
+    ppo_ubms_cvbs(m, tmin, Y, &hmin, &c);
+    ppo_ubms_cvbs(m, tmax, Y, &hmax, &c);
+  }
+  if (hv < hmin) {
+    if (use_h) ppo_hbms_cpbs(m, tmin, Y, &h1, &c); else ppo_ubms_cvbs(m, tmin, Y, &h1, &c);
+    if (ierr) *ierr = 1;
+    return tmin - (hmin - hv) / c;
+  }
+  if (hv > hmax) {
+    if (use_h) ppo_hbms_cpbs(m, tmax, Y, &h1, &c); else ppo_ubms_cvbs(m, tmax, Y, &h1, &c);
+    if (ierr) *ierr = 1;
+    return tmax - (hmax - hv) / c;
+  }
+  t1 = t;
+  if (t1 < tmin || t1 > tmax) t1 = tmin + (tmax - tmin) / (hmax - hmin) * (hv - hmin);
+  for (int i = 0; i < maxiter; ++i) {
+    if (use_h) ppo_hbms_cpbs(m, t1, Y, &h1, &c); else ppo_ubms_cvbs(m, t1, Y, &h1, &c);
+    dt = (hv - h1) / c;
+    if (dt > 100.) dt = 100.;
+    else if (dt < -100.) dt = -100.;
+    else if (fabs(dt) < tol) break;
+    t1 += dt;
+  }
+  if (ierr) *ierr = 0;
+  return t1;
+}
+
+double ppo_T_given_hy(const ppo_mech* m, double h, const double* Y, double Tg, int* ierr) {
+  return T_given(m, 1, h, Y, Tg, ierr);
+}
+double ppo_T_given_ey(const ppo_mech* m, double e, const double* Y, double Tg, int* ierr) {
+  return T_given(m, 0, e, Y, Tg, ierr);
+}
+
+/* ------------------------------------------------------------- kinetics -- */
+static double ipow(double x, double nu) {
+  int n = (int)lround(nu);
+  double r = 1.0;
+  for (int i = 0; i < n; ++i) r *= x;
+  return r;
+}
+
+/* d(prod_k c_k^nu_k)/dc_s for a sparse list */
+static double dprod(const int* ix, const double* v, int cnt, const double* sc, int s) {
+  double r = 1.0;
+  int found = 0;
+  for (int i = 0; i < cnt; ++i) {
+    if (ix[i] == s) {
+      found = 1;
+      r *= v[i] * ipow(sc[s], v[i] - 1.0);
+    } else {
+      r *= ipow(sc[ix[i]], v[i]);
+    }
+  }
+  return found ? r : 0.0;
+}
+
+static double prod_list(const int* ix, const double* v, int cnt, const double* sc) {
+  double r = 1.0;
+  for (int i = 0; i < cnt; ++i) r *= ipow(sc[ix[i]], v[i]);
+  return r;
+}
+
+/* Troe or Lindemann broadening factor F(T, Pr) in log10 form. */
+static double falloff_F(const ppo_mech* m, int j, double T, double Pr) {
+  if (m->troe_n[j] == 0) return 1.0;
+  const double* t = m->troe + 4 * j;
+  double Fcent = (1.0 - t[0]) * exp(-T / t[1]) + t[0] * exp(-T / t[2]);
+  if (m->troe_n[j] == 4) Fcent += exp(-t[3] / T);
+  double logFcent = log10(Fcent);
+  double c = -0.4 - 0.67 * logFcent;
+  double n = 0.75 - 1.27 * logFcent;
+  double x = (log10(Pr) + c) / (n - 0.14 * (log10(Pr) + c));
+  return pow(10.0, logFcent / (1.0 + x * x));
+}
+
+static void rates(const ppo_mech* m, const double* sc, double T, double* wdot, double* qout) {
+  const int ns = m->ns;
+  const double invT = 1.0 / T, logT = log(T);
+  const double refC = 101325.0 / 8.31446 * invT; /* truncated R of the reference */
+  double mixture = 0.0;
+  double gbuf[128];
+  double* g = ns > 128 ? (double*)malloc(sizeof(double) * ns) : gbuf;
+  for (int k = 0; k < ns; ++k) mixture += sc[k];
+  ppo_g_RT(m, T, g);
+  if (wdot) for (int k = 0; k < ns; ++k) wdot[k] = 0.0;
+  for (int j = 0; j < m->nr; ++j) {
+    const double* a = m->arr + 3 * j;
+    double kf = a[0] * exp(a[1] * logT - a[2] * invT);
+    double corr = 1.0;
+    if (m->kind[j] >= 1) {
+      double alpha = 0.0;
+      for (int k = 0; k < ns; ++k) alpha += m->eff[(size_t)j * ns + k] * sc[k];
+      (void)mixture;
+      if (m->kind[j] == 1) {
+        corr = alpha;
+      } else {
+        const double* al = m->arr_low + 3 * j;
+        double k0 = al[0] * exp(al[1] * logT - al[2] * invT);
+        double Pr = alpha * k0 / kf;
+        corr = Pr / (1.0 + Pr) * falloff_F(m, j, T, Pr);
+      }
+    }
+    double q = corr * kf * prod_list(m->ir + j * MAXSP, m->vr + j * MAXSP, m->nr_r[j], sc);
+    if (m->rev[j]) {
+      double dg = 0.0;
+      for (int i = 0; i < m->nr_r[j]; ++i) dg += m->vr[j * MAXSP + i] * g[m->ir[j * MAXSP + i]];
+      for (int i = 0; i < m->nr_p[j]; ++i) dg -= m->vp[j * MAXSP + i] * g[m->ip[j * MAXSP + i]];
+      double Kc = exp(dg) * pow(refC, -m->delta[j]);
+      q -= corr * kf / Kc * prod_list(m->ip + j * MAXSP, m->vp + j * MAXSP, m->nr_p[j], sc);
+    }
+    q *= m->scale[j];
+    if (qout) qout[j] = q;
+    if (wdot)
+      for (int i = 0; i < m->nr_n[j]; ++i) wdot[m->in[j * MAXSP + i]] += m->vn[j * MAXSP + i] * q;
+  }
+  if (g != gbuf) free(g);
+}
+
+void ppo_production_rate(const ppo_mech* m, const double* sc, double T, double* wdot) {
+  rates(m, sc, T, wdot, NULL);
+}
+void ppo_progress_rates(const ppo_mech* m, const double* sc, double T, double* q) {
+  rates(m, sc, T, NULL, q);
+}
+
+/* aJacobian semantics (Mechanisms/LiDryer/mechanism.H:3905-5138): exact
+ * derivatives, except that with consP == 1 the third-body/falloff terms use
+ * d(alpha)/dc_k = eff_k - 1 (the mixture-concentration derivative dropped). */
+void ppo_jacobian(const ppo_mech* m, const double* sc, double T, int consP, double* J) {
+  const int ns = m->ns, n = ns + 1;
+  const double invT = 1.0 / T, invT2 = invT * invT, logT = log(T);
+  const double refC = 101325.0 / 8.31446 * invT;
+  double *g = (double*)malloc(sizeof(double) * ns * 6);
+  double *hrt = g + ns, *wdot = g + 2 * ns, *cR = g + 3 * ns, *dcR = g + 4 * ns,
+         *eh = g + 5 * ns;
+  memset(J, 0, sizeof(double) * n * n);
+  for (int k = 0; k < ns; ++k) wdot[k] = 0.0;
+  ppo_g_RT(m, T, g);
+  ppo_h_RT(m, T, hrt);
+  for (int j = 0; j < m->nr; ++j) {
+    const int* ir = m->ir + j * MAXSP;
+    const int* ip = m->ip + j * MAXSP;
+    const double* vr = m->vr + j * MAXSP;
+    const double* vp = m->vp + j * MAXSP;
+    const double* a = m->arr + 3 * j;
+    const double phi_f = prod_list(ir, vr, m->nr_r[j], sc);
+    double k_f = a[0] * exp(a[1] * logT - a[2] * invT);
+    const double dlnkfdT = a[1] * invT + a[2] * invT2;
+    double alpha = 0.0, Pr = 0.0, fPr = 1.0, F = 1.0, dlogFdlogPr = 0.0, dlogFdT = 0.0,
+           dlogfPrdT = 0.0;
+    if (m->kind[j] >= 1) {
+      for (int k = 0; k < ns; ++k) alpha += m->eff[(size_t)j * ns + k] * sc[k];
+    }
+    if (m->kind[j] == 2) {
+      const double* al = m->arr_low + 3 * j;
+      double k0 = al[0] * exp(al[1] * logT - al[2] * invT);
+      Pr = alpha / k_f * k0;
+      fPr = Pr / (1.0 + Pr);
+      double dlnk0dT = al[1] * invT + al[2] * invT2;
+      double dlogPrdT = LOG10E * (dlnk0dT - dlnkfdT);
+      dlogfPrdT = dlogPrdT / (1.0 + Pr);
+      if (m->troe_n[j] > 0) {
+        const double* t = m->troe + 4 * j;
+        double logPr = log10(Pr);
+        double F1 = (1.0 - t[0]) * exp(-T / t[1]);
+        double F2 = t[0] * exp(-T / t[2]);
+        double F3 = m->troe_n[j] == 4 ? exp(-t[3] * invT) : 0.0;
+        double Fcent = F1 + F2 + F3;
+        double logFcent = log10(Fcent);
+        double tc = -0.4 - 0.67 * logFcent, tn = 0.75 - 1.27 * logFcent;
+        double den = 1.0 / (tn - 0.14 * (tc + logPr));
+        double tPr = (tc + logPr) * den;
+        double troe = 1.0 / (1.0 + tPr * tPr);
+        F = exp(LN10 * logFcent * troe);
+        double dlogFcentdT = LOG10E / Fcent *
+            (-F1 / t[1] - F2 / t[2] + (m->troe_n[j] == 4 ? F3 * t[3] * invT2 : 0.0));
+        double fac = 2.0 * logFcent * troe * troe * tPr * den;
+        double dlogFdc = -tn * fac * den;
+        double dlogFdn = fac * tPr;
+        dlogFdlogPr = dlogFdc;
+        dlogFdT = dlogFcentdT * (troe - 0.67 * dlogFdc - 1.27 * dlogFdn) + dlogFdlogPr * dlogPrdT;
+      }
+    }
+    double phi_r = 0.0, k_r = 0.0, dkrdT = 0.0;
+    if (m->rev[j]) {
+      phi_r = prod_list(ip, vp, m->nr_p[j], sc);
+      double dg = 0.0, hf = 0.0, hr = 0.0;
+      for (int i = 0; i < m->nr_r[j]; ++i) { dg += vr[i] * g[ir[i]]; hf += vr[i] * hrt[ir[i]]; }
+      for (int i = 0; i < m->nr_p[j]; ++i) { dg -= vp[i] * g[ip[i]]; hr += vp[i] * hrt[ip[i]]; }
+      k_r = k_f * exp(-dg) * pow(refC, m->delta[j]);
+      double dlnKcdT = invT * (-hf + hr + m->delta[j]);
+      dkrdT = (dlnkfdT - dlnKcdT) * k_r;
+    }
+    double q, dqdT, fac = 0.0;
+    int has_fac = 0;
+    if (m->kind[j] == 0) {
+      q = k_f * phi_f - k_r * phi_r;
+      dqdT = dlnkfdT * k_f * phi_f - dkrdT * phi_r;
+    } else if (m->kind[j] == 1) {
+      double qn = k_f * phi_f - k_r * phi_r;
+      q = alpha * qn;
+      dqdT = alpha * (dlnkfdT * k_f * phi_f - dkrdT * phi_r);
+      k_f *= alpha;
+      k_r *= alpha;
+      fac = qn;
+      has_fac = 1;
+    } else {
+      double qn = k_f * phi_f - k_r * phi_r;
+      double Corr = fPr * F;
+      q = Corr * qn;
+      double dlnCorrdT = LN10 * (dlogfPrdT + dlogFdT);
+      dqdT = Corr * (dlnkfdT * k_f * phi_f - dkrdT * phi_r) + dlnCorrdT * q;
+      k_f *= Corr;
+      k_r *= Corr;
+      fac = q / alpha * (1.0 / (Pr + 1.0) + dlogFdlogPr);
+      has_fac = 1;
+    }
+    const double s = m->scale[j];
+    q *= s; dqdT *= s; k_f *= s; k_r *= s; fac *= s;
+    for (int i = 0; i < m->nr_n[j]; ++i) wdot[m->in[j * MAXSP + i]] += m->vn[j * MAXSP + i] * q;
+    for (int k = 0; k < ns; ++k) {
+      double d = k_f * dprod(ir, vr, m->nr_r[j], sc, k);
+      if (m->rev[j]) d -= k_r * dprod(ip, vp, m->nr_p[j], sc, k);
+      if (has_fac) {
+        double e = m->eff[(size_t)j * ns + k];
+        d += (consP == 1 ? e - 1.0 : e) * fac;
+      }
+      if (d != 0.0)
+        for (int i = 0; i < m->nr_n[j]; ++i)
+          J[k * n + m->in[j * MAXSP + i]] += m->vn[j * MAXSP + i] * d;
+    }
+    for (int i = 0; i < m->nr_n[j]; ++i) J[ns * n + m->in[j * MAXSP + i]] += m->vn[j * MAXSP + i] * dqdT;
+  }
+  /* temperature row (mechanism.H:5098-5137) */
+  ppo_cp_R(m, T, cR);
+  if (consP != 1) for (int k = 0; k < ns; ++k) cR[k] -= 1.0;
+  dcpR_dT(m, T, dcR);
+  for (int k = 0; k < ns; ++k) eh[k] = (consP == 1) ? hrt[k] : hrt[k] - 1.0;
+  double cmix = 0.0, ehmix = 0.0, dcmixdT = 0.0, dehmixdT = 0.0;
+  for (int k = 0; k < ns; ++k) {
+    cmix += cR[k] * sc[k];
+    dcmixdT += dcR[k] * sc[k];
+    ehmix += eh[k] * wdot[k];
+    dehmixdT += invT * (cR[k] - eh[k]) * wdot[k] + eh[k] * J[ns * n + k];
+  }
+  double cmixinv = 1.0 / cmix, tmp1 = ehmix * cmixinv, tmp3 = cmixinv * T, tmp2 = tmp1 * tmp3;
+  for (int k = 0; k < ns; ++k) {
+    double d = 0.0;
+    for (int mm = 0; mm < ns; ++mm) d += eh[mm] * J[k * n + mm];
+    J[k * n + ns] = tmp2 * cR[k] - tmp3 * d;
+  }
+  J[n * n - 1] = -tmp1 + tmp2 * dcmixdT - tmp3 * dehmixdT;
+  free(g);
+}
+
+void ppo_rty2wdot(const ppo_mech* m, double rho, double T, const double* Y, double* wdot) {
+  double* c = (double*)malloc(sizeof(double) * m->ns);
+  for (int k = 0; k < m->ns; ++k) c[k] = rho * Y[k] * m->imw[k] * 1.0e6; /* CKYTCR, CKWC */
+  ppo_production_rate(m, c, T, wdot);
+  for (int k = 0; k < m->ns; ++k) wdot[k] *= 1.0e-6 * m->mw[k];
+  free(c);
+}
+
+void ppo_dwdot(const ppo_mech* m, const double* sc_cgs, double T, int consP, double* J) {
+  const int ns = m->ns, n = ns + 1;
+  double* c = (double*)malloc(sizeof(double) * ns);
+  for (int k = 0; k < ns; ++k) c[k] = 1.e6 * sc_cgs[k];
+  ppo_jacobian(m, c, T, consP, J);
+  for (int k = 0; k < ns; ++k) {
+    J[ns * n + k] *= 1.e-6;
+    J[k * n + ns] *= 1.e6;
+  }
+  free(c);
+}
+
+/* ---------------------------------------------------------- fKernelSpec -- */
+void ppo_fkernelspec(const ppo_mech* m, double dt_save, int reactor_type, const double* y,
+                     double* ydot, double rhoe_init, double rhoesrc, const double* rYsrc) {
+  const int ns = m->ns;
+  double* Y = (double*)malloc(sizeof(double) * ns * 3);
+  double *ei = Y + ns, *cd = Y + 2 * ns;
+  double rho = 0.0;
+  for (int k = 0; k < ns; ++k) rho += y[k];
+  double rinv = 1.0 / rho;
+  for (int k = 0; k < ns; ++k) Y[k] = y[k] * rinv;
+  double nrg = (rhoe_init + rhoesrc * dt_save) * rinv;
+  double T = y[ns], C = 1.0, dummy;
+  if (reactor_type == 1) {
+    T = ppo_T_given_ey(m, nrg, Y, T, NULL);
+    ppo_h_RT(m, T, ei);
+    for (int k = 0; k < ns; ++k) ei[k] = (ei[k] - 1.0) * RU * T * m->imw[k]; /* CKUMS */
+    ppo_ubms_cvbs(m, T, Y, &dummy, &C);
+  } else {
+    T = ppo_T_given_hy(m, nrg, Y, T, NULL);
+    ppo_h_RT(m, T, ei);
+    for (int k = 0; k < ns; ++k) ei[k] = ei[k] * RU * T * m->imw[k]; /* CKHMS */
+    ppo_hbms_cpbs(m, T, Y, &dummy, &C);
+  }
+  ppo_rty2wdot(m, rho, T, Y, cd);
+  double src = rhoesrc;
+  for (int k = 0; k < ns; ++k) {
+    double v = cd[k] + rYsrc[k];
+    ydot[k] = v;
+    src -= v * ei[k];
+  }
+  ydot[ns] = src * (rinv / C);
+  free(Y);
+}
+
+void ppo_chem_jac(const ppo_mech* m, int reactor_type, const double* y, double* Jt) {
+  const int ns = m->ns, n = ns + 1;
+  double* J = (double*)malloc(sizeof(double) * (n * n + ns));
+  double* c = J + n * n;
+  double rho = 0.0;
+  for (int k = 0; k < ns; ++k) rho += y[k];
+  for (int k = 0; k < ns; ++k) c[k] = rho * (y[k] / rho) * m->imw[k]; /* CKYTCR */
+  ppo_dwdot(m, c, y[ns], reactor_type == 2, J);
+  for (int i = 0; i < ns; ++i) {
+    for (int j = 0; j < ns; ++j) Jt[i * n + j] = J[j * n + i] * m->mw[i] * m->imw[j];
+    Jt[i * n + ns] = J[ns * n + i] * m->mw[i];
+    Jt[ns * n + i] = J[i * n + ns] * m->imw[i];
+  }
+  Jt[ns * n + ns] = J[n * n - 1];
+  free(J);
+}
+
+/* ---------------------------------------------------- generic systems -- */
+typedef struct {
+  int n, kind; /* 0 chemistry, 1 linear */
+  const ppo_mech* m;
+  int reactor_type;
+  double rhoe_init, rhoesrc;
+  const double* rYsrc;
+  const double *A, *b;
+  int nrhs, njac;
+} sys_t;
+
+static void sys_rhs(sys_t* s, double trel, const double* y, double* f) {
+  s->nrhs++;
+  if (s->kind == 0) {
+    ppo_fkernelspec(s->m, trel, s->reactor_type, y, f, s->rhoe_init, s->rhoesrc, s->rYsrc);
+  } else {
+    for (int i = 0; i < s->n; ++i) {
+      double v = s->b ? s->b[i] : 0.0;
+      for (int j = 0; j < s->n; ++j) v += s->A[i * s->n + j] * y[j];
+      f[i] = v;
+    }
+  }
+}
+
+static void sys_jac(sys_t* s, const double* y, double* Jt) {
+  s->njac++;
+  if (s->kind == 0) ppo_chem_jac(s->m, s->reactor_type, y, Jt);
+  else memcpy(Jt, s->A, sizeof(double) * s->n * s->n);
+}
+
+/* ------------------------------------------- ReactorBDF (parity) ------- */
+/* BDFParams (ReactorBDF.H:20-31): 4-decimal truncated coefficients. */
+static const double TCOEFF[4][4] = {{1.0000, -1.0000, 0.0000, 0.0000},
+                                    {1.0000, -1.0000, 0.0000, 0.0000},
+                                    {1.0000, -1.3333, 0.3333, 0.0000},
+                                    {1.0000, -1.6363, 0.8181, -0.1818}};
+static const double FCOEFF[4][4] = {{1.0000, 0.0000, 0.0000, 0.0000},
+                                    {0.5000, 0.5000, 0.0000, 0.0000},
+                                    {0.6666, 0.0000, 0.0000, 0.0000},
+                                    {0.5454, 0.0000, 0.0000, 0.0000}};
+
+/* Preconditioners (ReactorBDFsolver.H:12-91), A row-major n x n. */
+static void precond(int n, double* MinvX, const double* A, const double* X, int type,
+                    double* w) {
+  if (type == 0) {
+    for (int i = 0; i < n; ++i) MinvX[i] = X[i];
+  } else if (type == 1) {
+    for (int i = 0; i < n; ++i) MinvX[i] = X[i] / A[i * n + i];
+  } else {
+    double *y_d = w, *y_dd = w + n, *dinv = w + 2 * n;
+    for (int i = 0; i < n; ++i) dinv[i] = 1.0 / A[i * n + i];
+    for (int i = 0; i < n; ++i) {
+      double sum = 0.0;
+      for (int j = 0; j < i; ++j) sum += A[i * n + j] * y_d[j];
+      y_d[i] = (X[i] - sum) * dinv[i];
+    }
+    for (int i = 0; i < n; ++i) y_dd[i] = A[i * n + i] * y_d[i];
+    for (int i = n - 1; i >= 0; --i) {
+      double sum = 0.0;
+      for (int j = n - 1; j > i; --j) sum += A[i * n + j] * MinvX[j];
+      MinvX[i] = (y_dd[i] - sum) * dinv[i];
+    }
+  }
+}
+
+static double vnorm(int n, const double* v) {
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s = s + v[i] * v[i];
+  return sqrt(s);
+}
+
+/* performgmres (ReactorBDFsolver.H:384-499): left-preconditioned restarted GMRES
+ * with warm start x0, Givens rotations, lucky breakdown at h <= 0.01 tol. */
+static int gmres(int n, const double* A, const double* b, double* x0, double* x,
+                 int ptype, int restarts, int kspiters, double tol, double* w) {
+  const int K = n; /* KSPSIZE */
+  double* V = w;               /* (K+1) x n */
+  double* H = V + (K + 1) * n; /* (K+1) x K */
+  double* cs = H + (K + 1) * K;
+  double* sn = cs + K;
+  double* be = sn + K;  /* K+1 */
+  double* yv = be + K + 1; /* K+1 */
+  double* r = yv + K + 1;
+  double* t1 = r + n;
+  double* t2 = t1 + n;
+  double* pw = t2 + n; /* 3n */
+  memset(V, 0, sizeof(double) * (K + 1) * n);
+  memset(H, 0, sizeof(double) * (K + 1) * K);
+  memset(cs, 0, sizeof(double) * K);
+  memset(sn, 0, sizeof(double) * K);
+  memset(be, 0, sizeof(double) * (K + 1));
+  memset(yv, 0, sizeof(double) * (K + 1));
+  int nan_seen = 0;
+  for (int i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int j = 0; j < n; ++j) s += A[i * n + j] * x0[j];
+    t1[i] = b[i] - s;
+  }
+  precond(n, r, A, t1, ptype, pw);
+  for (int i = 0; i < n; ++i) x[i] = x0[i];
+  double residnorm = 0.0;
+  int lucky = 0;
+  for (int it = 0; it < restarts; ++it) {
+    for (int i = 0; i < n; ++i) x0[i] = x[i];
+    double beta = vnorm(n, r);
+    for (int i = 0; i < n; ++i) V[i] = r[i] / beta;
+    be[0] = beta;
+    int opt = K;
+    for (int k = 0; k < kspiters; ++k) {
+      for (int i = 0; i < n; ++i) x[i] = x0[i];
+      /* arnoldi (ReactorBDFsolver.H:258-317) */
+      const double* vj = V + k * n;
+      for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) s += A[i * n + j] * vj[j];
+        t1[i] = s;
+      }
+      precond(n, t2, A, t1, ptype, pw);
+      for (int i = 0; i <= k; ++i) {
+        double s = 0.0;
+        for (int l = 0; l < n; ++l) s = s + t2[l] * V[i * n + l];
+        H[i * K + k] = s;
+      }
+      for (int i = 0; i <= k; ++i)
+        for (int l = 0; l < n; ++l) t2[l] = 1.0 * t2[l] + (-H[i * K + k]) * V[i * n + l];
+      H[(k + 1) * K + k] = vnorm(n, t2);
+      lucky = 0;
+      if (H[(k + 1) * K + k] > tol * 0.01) {
+        for (int l = 0; l < n; ++l) V[(k + 1) * n + l] = t2[l] / H[(k + 1) * K + k];
+      } else {
+        double h = H[(k + 1) * K + k];
+        if (h == h && !isinf(h)) lucky = 1;
+        else nan_seen = 1;
+      }
+      /* triangularize (ReactorBDFsolver.H:331-366) */
+      for (int i = 0; i <= k - 1; ++i) {
+        double tmp = cs[i] * H[i * K + k] + sn[i] * H[(i + 1) * K + k];
+        H[(i + 1) * K + k] = -sn[i] * H[i * K + k] + cs[i] * H[(i + 1) * K + k];
+        H[i * K + k] = tmp;
+      }
+      double rr = H[k * K + k], ss = H[(k + 1) * K + k], c, sg;
+      if (rr == 0) { c = 0.0; sg = 1.0; }
+      else { c = rr / sqrt(rr * rr + ss * ss); sg = ss / sqrt(rr * rr + ss * ss); }
+      H[k * K + k] = c * H[k * K + k] + sg * H[(k + 1) * K + k];
+      H[(k + 1) * K + k] = 0.0;
+      cs[k] = c;
+      sn[k] = sg;
+      be[k + 1] = -sg * be[k];
+      be[k] = c * be[k];
+      residnorm = fabs(be[k + 1]);
+      if (nan_seen) return -1;
+      if (lucky) { opt = k + 1; break; }
+      if (residnorm <= tol) { opt = k + 1; break; }
+    }
+    /* uppertrisolve (ReactorBDFsolver.H:368-382) */
+    for (int i = opt - 1; i >= 0; --i) {
+      yv[i] = be[i];
+      for (int j = i + 1; j <= opt; ++j) yv[i] = yv[i] - (j < K ? H[i * K + j] : 0.0) * yv[j];
+      yv[i] = yv[i] / H[i * K + i];
+    }
+    for (int i = 0; i < opt; ++i)
+      for (int l = 0; l < n; ++l) x[l] = 1.0 * x[l] + yv[i] * V[i * n + l];
+    if (residnorm <= tol || lucky) break;
+  }
+  return 0;
+}
+
+typedef struct {
+  int n;
+  double *soln, *soln_n, *nm1, *nm2, *ds, *ds0, *rhs, *A, *Jt, *f, *fn, *gw;
+} bdf_ws;
+
+static void bdf_ws_init(bdf_ws* w, int n) {
+  w->n = n;
+  size_t gsz = (size_t)(n + 1) * n * 2 + 12 * (size_t)n + 16;
+  w->soln = (double*)calloc((size_t)9 * n + 2 * (size_t)n * n + gsz, sizeof(double));
+  w->soln_n = w->soln + n; w->nm1 = w->soln_n + n; w->nm2 = w->nm1 + n;
+  w->ds = w->nm2 + n; w->ds0 = w->ds + n; w->rhs = w->ds0 + n; w->f = w->rhs + n;
+  w->fn = w->f + n; w->A = w->fn + n; w->Jt = w->A + (size_t)n * n; w->gw = w->Jt + (size_t)n * n;
+}
+static void bdf_ws_free(bdf_ws* w) { free(w->soln); }
+
+/* Fixed-step BDF core of ReactorBDF::react (ReactorBDF.cpp:187-285): y (n) in/out. */
+static int bdf_core(sys_t* s, const ppo_bdf_params* p, double* y, double dt_react, bdf_ws* w) {
+  const int n = s->n;
+  const double dt = dt_react / (double)p->nsubsteps;
+  const double dt_inv = 1.0 / dt;
+  double current_time = 0.0; /* relative to time_init */
+  memcpy(w->soln_n, y, sizeof(double) * n);
+  memcpy(w->soln, y, sizeof(double) * n);
+  memcpy(w->nm1, y, sizeof(double) * n);
+  memcpy(w->nm2, y, sizeof(double) * n);
+  memset(w->ds, 0, sizeof(double) * n);
+  memset(w->ds0, 0, sizeof(double) * n);
+  const int first = (p->scheme >= 2) ? 1 : p->scheme;
+  const int change = p->scheme - 2;
+  for (int st = 0; st < p->nsubsteps; ++st) {
+    const int ts = (st > change) ? p->scheme : first;
+    memset(w->ds0, 0, sizeof(double) * n);
+    for (int it = 0; it < p->nonlinear_iters; ++it) {
+      memcpy(w->ds0, w->ds, sizeof(double) * n);
+      /* get_bdf_matrix_and_rhs (ReactorBDF.cpp:36-110) */
+      sys_jac(s, w->soln, w->Jt);
+      for (int i = 0; i < n * n; ++i) w->A[i] = -FCOEFF[ts][0] * w->Jt[i];
+      sys_rhs(s, current_time, w->soln, w->f);
+      if (ts == 1) sys_rhs(s, current_time, w->soln_n, w->fn);
+      else memset(w->fn, 0, sizeof(double) * n);
+      for (int i = 0; i < n; ++i) {
+        w->A[i * n + i] += TCOEFF[ts][0] * dt_inv;
+        double r = -TCOEFF[ts][0] * w->soln[i] * dt_inv;
+        r += -TCOEFF[ts][1] * w->soln_n[i] * dt_inv;
+        r += -TCOEFF[ts][2] * w->nm1[i] * dt_inv;
+        r += -TCOEFF[ts][3] * w->nm2[i] * dt_inv;
+        r += FCOEFF[ts][0] * w->f[i];
+        r += FCOEFF[ts][1] * w->fn[i];
+        w->rhs[i] = r;
+      }
+      if (gmres(n, w->A, w->rhs, w->ds0, w->ds, p->gmres_precond, p->gmres_restarts,
+                p->gmres_kspiters, p->gmres_tol, w->gw) != 0)
+        return -1;
+      for (int i = 0; i < n; ++i) w->soln[i] += w->ds[i];
+      double nrm = 0.0;
+      for (int i = 0; i < n; ++i) nrm += w->rhs[i] * w->rhs[i];
+      if (sqrt(nrm) <= p->nonlin_tol) break;
+    }
+    for (int i = 0; i < n; ++i) {
+      w->nm2[i] = w->nm1[i];
+      w->nm1[i] = w->soln_n[i];
+      w->soln_n[i] = w->soln[i];
+    }
+    current_time += dt;
+  }
+  memcpy(y, w->soln_n, sizeof(double) * n);
+  return p->nsubsteps;
+}
+
+/* ----------------------------------------------- ReactorRK64 (parity) -- */
+/* RK64Params (ReactorRK64.H:9-40) */
+static const double RK_ALPHA[6] = {0.218150805229859, 0.256702469801519, 0.527402592007520,
+                                   0.0484864267224467, 1.24517071533530, 0.412366034843237};
+static const double RK_BETA[6] = {-0.113554138044166, -0.215118587818400, -0.0510152146250577,
+                                  -1.07992686223881, -0.248664241213447, 0.0};
+static const double RK_ERR[6] = {-0.0554699315064507, 0.158481845574980, -0.0905918835751907,
+                                 -0.219084567203338, 0.164022338959433, 0.0426421977505659};
+
+/* RK64 core (ReactorRK64.cpp:73-144); returns nsteps. */
+static int rk64_core(sys_t* s, const ppo_rk64_params* p, double* y, double dt_react, double* w) {
+  const int n = s->n;
+  double *carry = w, *err = w + n, *yd = w + 2 * n;
+  const double time_init = 0.0, time_out = dt_react;
+  double current_time = time_init;
+  for (int i = 0; i < n; ++i) carry[i] = y[i];
+  double dt_rk = dt_react / (double)p->nsub_guess;
+  const double dt_min = dt_react / (double)p->nsub_max;
+  const double dt_max = dt_react / (double)p->nsub_min;
+  int nsteps = 0;
+  while (current_time < time_out) {
+    for (int i = 0; i < n; ++i) err[i] = 0.0;
+    for (int st = 0; st < 6; ++st) {
+      sys_rhs(s, current_time - time_init, y, yd);
+      for (int i = 0; i < n; ++i) {
+        err[i] += RK_ERR[st] * dt_rk * yd[i];
+        y[i] = carry[i] + RK_ALPHA[st] * dt_rk * yd[i];
+        carry[i] = y[i] + RK_BETA[st] * dt_rk * yd[i];
+      }
+    }
+    current_time += dt_rk;
+    nsteps++;
+    double max_err = 1e-50;
+    for (int i = 0; i < n; ++i) max_err = fabs(err[i]) > max_err ? fabs(err[i]) : max_err;
+    if (max_err < p->atol) {
+      double cf = 1.0 * pow(p->atol / max_err, 0.25);
+      dt_rk = fmin(dt_max, dt_rk * cf);
+    } else {
+      double cf = 1.0 * pow(p->atol / max_err, 0.2);
+      dt_rk = fmax(dt_min, dt_rk * cf);
+    }
+    dt_rk = fmin(dt_rk, time_out - current_time);
+  }
+  return nsteps;
+}
+
+/* --------------------------------------- adaptive BDF (NDF family) ----- */
+/* Variable-order (1-5), quasi-constant step size backward-difference method of
+ * Shampine & Reichelt, "The MATLAB ODE Suite", SIAM J. Sci. Comput. 18 (1997),
+ * with the NDF kappa coefficients; the same algorithm as SciPy's
+ * solve_ivp(method="BDF"): modified Newton (<= 4 iterations), Jacobian
+ * refreshed only after a Newton failure, RMS error norm with
+ * scale = atol + rtol |y|, step factor limits [0.2, 10], order change after
+ * order+1 equal steps. */
+#define AMAXORD 5
+static const double NDF_KAPPA[6] = {0, -0.1850, -1.0 / 9.0, -0.0823, -0.0415, 0};
+
+typedef struct {
+  int n;
+  double *D, *J, *LU, *ypred, *scale, *psi, *y, *d, *f, *dy, *rhs, *tmp;
+  int* piv;
+} ad_ws;
+
+static void ad_ws_init(ad_ws* w, int n) {
+  w->n = n;
+  w->D = (double*)calloc((size_t)(AMAXORD + 3) * n + 2 * (size_t)n * n + 10 * (size_t)n, sizeof(double));
+  w->J = w->D + (AMAXORD + 3) * n;
+  w->LU = w->J + (size_t)n * n;
+  w->ypred = w->LU + (size_t)n * n;
+  w->scale = w->ypred + n; w->psi = w->scale + n; w->y = w->psi + n; w->d = w->y + n;
+  w->f = w->d + n; w->dy = w->f + n; w->rhs = w->dy + n; w->tmp = w->rhs + n;
+  w->piv = (int*)calloc(n, sizeof(int));
+}
+static void ad_ws_free(ad_ws* w) { free(w->D); free(w->piv); }
+
+static double rms(int n, const double* x, const double* sc) {
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) { double v = x[i] / sc[i]; s += v * v; }
+  return sqrt(s / n);
+}
+
+/* LU with partial pivoting (getrf) of the row-major matrix a. */
+static int lu_factor(int n, double* a, int* piv) {
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double mx = fabs(a[k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(a[i * n + k]) > mx) { mx = fabs(a[i * n + k]); p = i; }
+    piv[k] = p;
+    if (p != k)
+      for (int j = 0; j < n; ++j) { double t = a[k * n + j]; a[k * n + j] = a[p * n + j]; a[p * n + j] = t; }
+    if (a[k * n + k] == 0.0) return -1;
+    double inv = 1.0 / a[k * n + k];
+    for (int i = k + 1; i < n; ++i) {
+      double l = a[i * n + k] * inv;
+      a[i * n + k] = l;
+      for (int j = k + 1; j < n; ++j) a[i * n + j] -= l * a[k * n + j];
+    }
+  }
+  return 0;
+}
+
+static void lu_solve(int n, const double* a, const int* piv, double* b) {
+  for (int k = 0; k < n; ++k)
+    if (piv[k] != k) { double t = b[k]; b[k] = b[piv[k]]; b[piv[k]] = t; }
+  for (int i = 0; i < n; ++i) {
+    double s = b[i];
+    for (int j = 0; j < i; ++j) s -= a[i * n + j] * b[j];
+    b[i] = s;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int j = i + 1; j < n; ++j) s -= a[i * n + j] * b[j];
+    b[i] = s / a[i * n + i];
+  }
+}
+
+/* D[:order+1] <- (R U)^T D[:order+1] with R = R(order, factor), U = R(order, 1). */
+static void change_D(int n, double* D, int order, double factor) {
+  double R[AMAXORD + 1][AMAXORD + 1], U[AMAXORD + 1][AMAXORD + 1], RUm[AMAXORD + 1][AMAXORD + 1];
+  const int o = order;
+  for (int pass = 0; pass < 2; ++pass) {
+    double f = pass == 0 ? factor : 1.0;
+    double (*M)[AMAXORD + 1] = pass == 0 ? R : U;
+    double Mr[AMAXORD + 1][AMAXORD + 1];
+    for (int i = 0; i <= o; ++i)
+      for (int j = 0; j <= o; ++j) Mr[i][j] = (i == 0) ? 1.0 : (j == 0 ? 0.0 : (i - 1 - f * j) / (double)i);
+    for (int j = 0; j <= o; ++j) {
+      double c = 1.0;
+      for (int i = 0; i <= o; ++i) { c *= Mr[i][j]; M[i][j] = c; }
+    }
+  }
+  for (int i = 0; i <= o; ++i)
+    for (int j = 0; j <= o; ++j) {
+      double s = 0.0;
+      for (int k = 0; k <= o; ++k) s += R[i][k] * U[k][j];
+      RUm[i][j] = s;
+    }
+  double tmp[AMAXORD + 1];
+  for (int c = 0; c < n; ++c) {
+    for (int i = 0; i <= o; ++i) {
+      double s = 0.0;
+      for (int k = 0; k <= o; ++k) s += RUm[k][i] * D[k * n + c];
+      tmp[i] = s;
+    }
+    for (int i = 0; i <= o; ++i) D[i * n + c] = tmp[i];
+  }
+}
+
+static int adaptive_core(sys_t* s, const ppo_adapt_params* p, double* y0, double t_bound,
+                         ad_ws* w, ppo_adapt_stats* st) {
+  const int n = s->n;
+  const double rtol = p->rtol, atol = p->atol;
+  const double EPS = 2.220446049250313e-16;
+  double gamma[AMAXORD + 1], alpha[AMAXORD + 1], econst[AMAXORD + 2];
+  gamma[0] = 0.0;
+  for (int i = 1; i <= AMAXORD; ++i) gamma[i] = gamma[i - 1] + 1.0 / i;
+  for (int i = 0; i <= AMAXORD; ++i) alpha[i] = (1.0 - NDF_KAPPA[i]) * gamma[i];
+  for (int i = 0; i <= AMAXORD; ++i) econst[i] = NDF_KAPPA[i] * gamma[i] + 1.0 / (i + 1);
+  econst[AMAXORD + 1] = 0.0;
+  const double newton_tol = fmax(10 * EPS / rtol, fmin(0.03, sqrt(rtol)));
+  double* D = w->D;
+  double t = 0.0;
+  int nsteps = 0, nlu = 0, nnewton = 0, nrej = 0, nfail = 0;
+  /* initial step (Hairer, Norsett, Wanner I, II.4) */
+  sys_rhs(s, t, y0, w->f);
+  double h_abs;
+  if (p->first_step > 0) {
+    h_abs = p->first_step;
+  } else {
+    for (int i = 0; i < n; ++i) w->scale[i] = atol + fabs(y0[i]) * rtol;
+    double d0 = rms(n, y0, w->scale), d1 = rms(n, w->f, w->scale);
+    double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+    h0 = fmin(h0, t_bound);
+    for (int i = 0; i < n; ++i) w->y[i] = y0[i] + h0 * w->f[i];
+    sys_rhs(s, t + h0, w->y, w->dy);
+    for (int i = 0; i < n; ++i) w->tmp[i] = w->dy[i] - w->f[i];
+    double d2 = rms(n, w->tmp, w->scale) / h0;
+    double h1 = (d1 <= 1e-15 && d2 <= 1e-15) ? fmax(1e-6, h0 * 1e-3)
+                                             : pow(0.01 / fmax(d1, d2), 1.0 / 2.0);
+    h_abs = fmin(fmin(100 * h0, h1), t_bound);
+  }
+  for (int i = 0; i < n; ++i) { D[i] = y0[i]; D[n + i] = w->f[i] * h_abs; }
+  int order = 1, n_equal = 0, lu_valid = 0;
+  sys_jac(s, y0, w->J);
+  int status = 0;
+  while (t < t_bound) {
+    if (p->max_steps > 0 && nsteps >= p->max_steps) { status = -2; break; }
+    double min_step = 10 * fabs(nextafter(t, INFINITY) - t);
+    if (h_abs < min_step) {
+      change_D(n, D, order, min_step / h_abs);
+      h_abs = min_step;
+      n_equal = 0;
+    }
+    int current_jac = 0, accepted = 0, n_iter = 0;
+    double t_new = t, error_norm = 0.0, safety = 0.9;
+    while (!accepted) {
+      if (h_abs < min_step) { status = -3; goto done; }
+      double h = h_abs;
+      t_new = t + h;
+      if (t_new - t_bound > 0) {
+        t_new = t_bound;
+        change_D(n, D, order, fabs(t_new - t) / h_abs);
+        n_equal = 0;
+        lu_valid = 0;
+      }
+      h = t_new - t;
+      h_abs = fabs(h);
+      for (int i = 0; i < n; ++i) {
+        double sum = 0.0;
+        for (int k = 0; k <= order; ++k) sum += D[k * n + i];
+        w->ypred[i] = sum;
+        w->scale[i] = atol + rtol * fabs(sum);
+        double ps = 0.0;
+        for (int k = 1; k <= order; ++k) ps += D[k * n + i] * gamma[k];
+        w->psi[i] = ps / alpha[order];
+      }
+      int converged = 0;
+      const double c = h / alpha[order];
+      while (!converged) {
+        if (!lu_valid) {
+          for (int i = 0; i < n * n; ++i) w->LU[i] = -c * w->J[i];
+          for (int i = 0; i < n; ++i) w->LU[i * n + i] += 1.0;
+          lu_factor(n, w->LU, w->piv);
+          nlu++;
+          lu_valid = 1;
+        }
+        /* solve_bdf_system */
+        for (int i = 0; i < n; ++i) { w->y[i] = w->ypred[i]; w->d[i] = 0.0; }
+        double dy_norm_old = -1.0;
+        converged = 0;
+        int k;
+        for (k = 0; k < 4; ++k) {
+          sys_rhs(s, t_new, w->y, w->f);
+          nnewton++;
+          int finite = 1;
+          for (int i = 0; i < n; ++i) if (!isfinite(w->f[i])) finite = 0;
+          if (!finite) break;
+          for (int i = 0; i < n; ++i) w->dy[i] = c * w->f[i] - w->psi[i] - w->d[i];
+          lu_solve(n, w->LU, w->piv, w->dy);
+          double dy_norm = rms(n, w->dy, w->scale);
+          double rate = dy_norm_old < 0 ? -1.0 : dy_norm / dy_norm_old;
+          if (rate >= 0 && (rate >= 1 || pow(rate, 4 - k) / (1 - rate) * dy_norm > newton_tol)) break;
+          for (int i = 0; i < n; ++i) { w->y[i] += w->dy[i]; w->d[i] += w->dy[i]; }
+          if (dy_norm == 0 || (rate >= 0 && rate / (1 - rate) * dy_norm < newton_tol)) {
+            converged = 1;
+            break;
+          }
+          dy_norm_old = dy_norm;
+        }
+        n_iter = k + 1;
+        if (!converged) {
+          if (current_jac) break;
+          sys_jac(s, w->ypred, w->J);
+          lu_valid = 0;
+          current_jac = 1;
+        }
+      }
+      if (!converged) {
+        h_abs *= 0.5;
+        change_D(n, D, order, 0.5);
+        n_equal = 0;
+        lu_valid = 0;
+        nfail++;
+        continue;
+      }
+      safety = 0.9 * (2 * 4 + 1) / (double)(2 * 4 + n_iter);
+      for (int i = 0; i < n; ++i) {
+        w->scale[i] = atol + rtol * fabs(w->y[i]);
+        w->tmp[i] = econst[order] * w->d[i];
+      }
+      error_norm = rms(n, w->tmp, w->scale);
+      if (error_norm > 1) {
+        double factor = fmax(0.2, safety * pow(error_norm, -1.0 / (order + 1)));
+        h_abs *= factor;
+        change_D(n, D, order, factor);
+        n_equal = 0;
+        nrej++;
+      } else {
+        accepted = 1;
+      }
+    }
+    nsteps++;
+    n_equal++;
+    t = t_new;
+    for (int i = 0; i < n; ++i) {
+      D[(order + 2) * n + i] = w->d[i] - D[(order + 1) * n + i];
+      D[(order + 1) * n + i] = w->d[i];
+    }
+    for (int k = order; k >= 0; --k)
+      for (int i = 0; i < n; ++i) D[k * n + i] += D[(k + 1) * n + i];
+    if (n_equal < order + 1) continue;
+    double em = INFINITY, ep = INFINITY;
+    if (order > 1) {
+      for (int i = 0; i < n; ++i) w->tmp[i] = econst[order - 1] * D[order * n + i];
+      em = rms(n, w->tmp, w->scale);
+    }
+    if (order < AMAXORD) {
+      for (int i = 0; i < n; ++i) w->tmp[i] = econst[order + 1] * D[(order + 2) * n + i];
+      ep = rms(n, w->tmp, w->scale);
+    }
+    double norms[3] = {em, error_norm, ep};
+    double best = -1.0;
+    int delta = 0;
+    for (int i = 0; i < 3; ++i) {
+      double fct = (norms[i] == 0.0) ? INFINITY : pow(norms[i], -1.0 / (order + i));
+      if (fct > best) { best = fct; delta = i - 1; }
+    }
+    order += delta;
+    double factor = fmin(10.0, safety * best);
+    h_abs *= factor;
+    change_D(n, D, order, factor);
+    n_equal = 0;
+    lu_valid = 0;
+  }
+done:
+  for (int i = 0; i < n; ++i) y0[i] = D[i];
+  if (st) {
+    st->nsteps += nsteps; st->nrhs += s->nrhs; st->njac += s->njac; st->nlu += nlu;
+    st->nnewton += nnewton; st->nreject += nrej; st->nfail += nfail;
+    if (status != 0) st->status = status;
+  }
+  return status == 0 ? nsteps : status;
+}
+
+/* ------------------------------------------------------------ drivers -- */
+static int threads_or_default(int nthreads) {
+#ifdef _OPENMP
+  return nthreads > 0 ? nthreads : omp_get_max_threads();
+#else
+  (void)nthreads;
+  return 1;
+#endif
+}
+
+int ppo_react_bdf_1d(const ppo_mech* m, const ppo_bdf_params* p, int reactor_type, double* y,
+                     const double* ysrc, double* rX, const double* rXsrc, double dt_react,
+                     double time, int ncells, int* FC, int nthreads) {
+  (void)time;
+  const int n = m->ns + 1;
+  long total = 0;
+  int bad = 0;
+#pragma omp parallel num_threads(threads_or_default(nthreads)) reduction(+ : total, bad)
+  {
+    bdf_ws w;
+    bdf_ws_init(&w, n);
+#pragma omp for schedule(dynamic, 64)
+    for (int c = 0; c < ncells; ++c) {
+      sys_t s = {n, 0, m, reactor_type, rX[c], rXsrc[c], ysrc + (size_t)c * m->ns, NULL, NULL, 0, 0};
+      int r = bdf_core(&s, p, y + (size_t)c * n, dt_react, &w);
+      if (r < 0) bad++;
+      else total += r;
+      if (FC) FC[c] = r;
+      rX[c] = rX[c] + dt_react * rXsrc[c];
+    }
+    bdf_ws_free(&w);
+  }
+  if (bad) return -1;
+  return (int)(total / (double)ncells);
+}
+
+int ppo_react_rk64_1d(const ppo_mech* m, const ppo_rk64_params* p, int reactor_type, double* y,
+                      const double* ysrc, double* rX, const double* rXsrc, double dt_react,
+                      double time, int ncells, int* FC, int nthreads) {
+  (void)time;
+  const int n = m->ns + 1;
+  long total = 0;
+#pragma omp parallel num_threads(threads_or_default(nthreads)) reduction(+ : total)
+  {
+    double* w = (double*)malloc(sizeof(double) * 3 * n);
+#pragma omp for schedule(dynamic, 64)
+    for (int c = 0; c < ncells; ++c) {
+      sys_t s = {n, 0, m, reactor_type, rX[c], rXsrc[c], ysrc + (size_t)c * m->ns, NULL, NULL, 0, 0};
+      int r = rk64_core(&s, p, y + (size_t)c * n, dt_react, w);
+      total += r;
+      if (FC) FC[c] = r;
+      rX[c] = rX[c] + dt_react * rXsrc[c];
+    }
+    free(w);
+  }
+  return (int)(total / (double)ncells);
+}
+
+int ppo_react_adaptive_1d(const ppo_mech* m, const ppo_adapt_params* p, int reactor_type,
+                          double* y, const double* ysrc, double* rX, const double* rXsrc,
+                          double dt_react, double time, int ncells, ppo_adapt_stats* stats,
+                          int nthreads) {
+  (void)time;
+  const int n = m->ns + 1;
+  long total = 0;
+  int bad = 0;
+  ppo_adapt_stats agg = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma omp parallel num_threads(threads_or_default(nthreads)) reduction(+ : total, bad)
+  {
+    ad_ws w;
+    ad_ws_init(&w, n);
+    ppo_adapt_stats loc = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma omp for schedule(dynamic, 64)
+    for (int c = 0; c < ncells; ++c) {
+      sys_t s = {n, 0, m, reactor_type, rX[c], rXsrc[c], ysrc + (size_t)c * m->ns, NULL, NULL, 0, 0};
+      ppo_adapt_stats one = {0, 0, 0, 0, 0, 0, 0, 0};
+      int r = adaptive_core(&s, p, y + (size_t)c * n, dt_react, &w, &one);
+      if (r < 0) bad++;
+      else total += r;
+      loc.nsteps += one.nsteps; loc.nrhs += one.nrhs; loc.njac += one.njac; loc.nlu += one.nlu;
+      loc.nnewton += one.nnewton; loc.nreject += one.nreject; loc.nfail += one.nfail;
+      if (one.status) loc.status = one.status;
+      rX[c] = rX[c] + dt_react * rXsrc[c];
+    }
+#pragma omp critical
+    {
+      agg.nsteps += loc.nsteps; agg.nrhs += loc.nrhs; agg.njac += loc.njac; agg.nlu += loc.nlu;
+      agg.nnewton += loc.nnewton; agg.nreject += loc.nreject; agg.nfail += loc.nfail;
+      if (loc.status) agg.status = loc.status;
+    }
+    ad_ws_free(&w);
+  }
+  if (stats) *stats = agg;
+  if (bad) return -1;
+  return (int)(total / (double)ncells);
+}
+
+int ppo_react_box(const ppo_mech* m, int mode, const ppo_bdf_params* pb,
+                  const ppo_rk64_params* pr, const ppo_adapt_params* pa, int reactor_type,
+                  double* rY, const double* rYsrc, double* T, double* rE, const double* rEsrc,
+                  double* FC, double dt_react, double time, int ncells, ppo_adapt_stats* stats,
+                  int nthreads) {
+  (void)time;
+  const int ns = m->ns, n = ns + 1;
+  long total = 0;
+  int bad = 0;
+#pragma omp parallel num_threads(threads_or_default(nthreads)) reduction(+ : total, bad)
+  {
+    bdf_ws wb;
+    ad_ws wa;
+    bdf_ws_init(&wb, n);
+    ad_ws_init(&wa, n);
+    double* y = (double*)malloc(sizeof(double) * (4 * n + 2 * ns));
+    double* src = y + n;
+    double* Y = src + ns;
+    double* rw = Y + ns;
+#pragma omp for schedule(dynamic, 64)
+    for (int c = 0; c < ncells; ++c) {
+      double rho = 0.0;
+      for (int k = 0; k < ns; ++k) { y[k] = rY[(size_t)k * ncells + c]; rho += y[k]; }
+      for (int k = 0; k < ns; ++k) { Y[k] = y[k] / rho; src[k] = rYsrc[(size_t)k * ncells + c]; }
+      double temp = T[c];
+      double enrg = rE[c] / rho;
+      temp = (reactor_type == 1) ? ppo_T_given_ey(m, enrg, Y, temp, NULL)
+                                 : ppo_T_given_hy(m, enrg, Y, temp, NULL);
+      y[ns] = temp;
+      sys_t s = {n, 0, m, reactor_type, rE[c], rEsrc[c], src, NULL, NULL, 0, 0};
+      int r;
+      if (mode == 0) r = bdf_core(&s, pb, y, dt_react, &wb);
+      else if (mode == 1) r = rk64_core(&s, pr, y, dt_react, rw);
+      else {
+        ppo_adapt_stats one = {0, 0, 0, 0, 0, 0, 0, 0};
+        r = adaptive_core(&s, pa, y, dt_react, &wa, &one);
+      }
+      if (r < 0) bad++;
+      else total += r;
+      rho = 0.0;
+      for (int k = 0; k < ns; ++k) rho += y[k];
+      for (int k = 0; k < ns; ++k) { Y[k] = y[k] / rho; rY[(size_t)k * ncells + c] = y[k]; }
+      temp = y[ns];
+      rE[c] = rE[c] + dt_react * rEsrc[c];
+      enrg = rE[c] / rho;
+      temp = (reactor_type == 1) ? ppo_T_given_ey(m, enrg, Y, temp, NULL)
+                                 : ppo_T_given_hy(m, enrg, Y, temp, NULL);
+      T[c] = temp;
+      FC[c] = (double)r;
+    }
+    free(y);
+    bdf_ws_free(&wb);
+    ad_ws_free(&wa);
+  }
+  (void)stats;
+  if (bad) return -1;
+  return (int)(total / (double)ncells);
+}
+
+int ppo_bdf_linear(int n, const double* A, const double* b, double* y, double dt_react,
+                   const ppo_bdf_params* p) {
+  sys_t s = {n, 1, NULL, 0, 0, 0, NULL, A, b, 0, 0};
+  bdf_ws w;
+  bdf_ws_init(&w, n);
+  int r = bdf_core(&s, p, y, dt_react, &w);
+  bdf_ws_free(&w);
+  return r;
+}
+
+int ppo_rk64_linear(int n, const double* A, const double* b, double* y, double dt_react,
+                    const ppo_rk64_params* p) {
+  sys_t s = {n, 1, NULL, 0, 0, 0, NULL, A, b, 0, 0};
+  double* w = (double*)malloc(sizeof(double) * 3 * n);
+  int r = rk64_core(&s, p, y, dt_react, w);
+  free(w);
+  return r;
+}
+
+int ppo_adaptive_linear(int n, const double* A, const double* b, double* y, double t_end,
+                        const ppo_adapt_params* p, ppo_adapt_stats* st) {
+  sys_t s = {n, 1, NULL, 0, 0, 0, NULL, A, b, 0, 0};
+  ad_ws w;
+  ad_ws_init(&w, n);
+  int r = adaptive_core(&s, p, y, t_end, &w, st);
+  ad_ws_free(&w);
+  return r;
+}

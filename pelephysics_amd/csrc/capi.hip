@@ -1,0 +1,404 @@
+// C-ABI of the chemistry integrators (include/pele_chem_hip.h): handle
+// management, reference defaults, host<->device staging for the 1-D form,
+// kernel dispatch by mechanism, and the step-count reduction that gives the
+// reference's int(average steps) return value.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pele_chem_hip.h"
+#include "launch.h"
+
+namespace ppc {
+#define PPC_MECH(M) const MechLauncher* launcher_##M();
+#include "inst/registry.inc"
+#undef PPC_MECH
+}  // namespace ppc
+
+namespace {
+
+using ppc::MechLauncher;
+
+const MechLauncher* find_mech(const char* name) {
+  const MechLauncher* all[] = {
+#define PPC_MECH(M) ppc::launcher_##M(),
+#include "inst/registry.inc"
+#undef PPC_MECH
+  };
+  for (const MechLauncher* m : all)
+    if (name && std::strcmp(m->name, name) == 0) return m;
+  return nullptr;
+}
+
+// Sums the per-cell counters and reduces min/max of the step counts.
+__global__ void __launch_bounds__(256) reduce_stats(const int* __restrict__ nsteps, const int* __restrict__ nrhs,
+                                                    const int* __restrict__ njac, const int* __restrict__ nlu,
+                                                    const int* __restrict__ nnewton,
+                                                    const int* __restrict__ nreject, int ncells,
+                                                    unsigned long long* __restrict__ out) {
+  // out: [0] sum steps, [1] rhs, [2] jac, [3] lu, [4] newton, [5] reject,
+  //      [6] failed cells, [7] max steps, [8] min steps
+  unsigned long long s[7] = {0, 0, 0, 0, 0, 0, 0};
+  int mx = 0, mn = 0x7fffffff;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x) {
+    const int v = nsteps[c];
+    if (v < 0) {
+      s[6] += 1;
+    } else {
+      s[0] += (unsigned long long)v;
+      mx = v > mx ? v : mx;
+      mn = v < mn ? v : mn;
+    }
+    if (nrhs) {
+      s[1] += (unsigned long long)nrhs[c];
+      s[2] += (unsigned long long)njac[c];
+      s[3] += (unsigned long long)nlu[c];
+      s[4] += (unsigned long long)nnewton[c];
+      s[5] += (unsigned long long)nreject[c];
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    for (int i = 0; i < 7; ++i) s[i] += __shfl_down(s[i], off, 64);
+    const int omx = __shfl_down(mx, off, 64);
+    const int omn = __shfl_down(mn, off, 64);
+    mx = omx > mx ? omx : mx;
+    mn = omn < mn ? omn : mn;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    for (int i = 0; i < 7; ++i)
+      if (s[i]) atomicAdd(&out[i], s[i]);
+    atomicMax(&out[7], (unsigned long long)mx);
+    atomicMin(&out[8], (unsigned long long)mn);
+  }
+}
+
+__global__ void init_stats(unsigned long long* out) {
+  if (threadIdx.x < 9) out[threadIdx.x] = (threadIdx.x == 8) ? 0x7fffffffULL : 0ULL;
+}
+
+}  // namespace
+
+struct pp_chem {
+  pp_chem_config cfg;
+  std::string mech_name;
+  const MechLauncher* L = nullptr;
+  ppc::ReactParamsDev P{};
+  int device = 0;
+  // per-cell counters (device), grown on demand
+  int cap = 0;
+  int* d_cnt = nullptr;  // 6 * cap ints
+  unsigned long long* d_red = nullptr;
+  // staging for host pointers in the 1-D form
+  size_t stage_cap = 0;
+  double* d_stage = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  pp_chem_stats stats{};
+  std::string err;
+};
+
+namespace {
+
+int fail(pp_chem* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+int hip_fail(pp_chem* h, hipError_t e, const char* where) {
+  return fail(h, PP_CHEM_EHIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(h, x)                                   \
+  do {                                                 \
+    hipError_t e_ = (x);                               \
+    if (e_ != hipSuccess) return hip_fail(h, e_, #x); \
+  } while (0)
+
+int ensure_counters(pp_chem* h, int ncells) {
+  if (ncells <= h->cap) return 0;
+  if (h->d_cnt) HIPCHK(h, hipFree(h->d_cnt));
+  h->d_cnt = nullptr;
+  HIPCHK(h, hipMalloc(&h->d_cnt, sizeof(int) * 6 * (size_t)ncells));
+  h->cap = ncells;
+  return 0;
+}
+
+ppc::CellStats counters(pp_chem* h) {
+  ppc::CellStats st;
+  const size_t c = (size_t)h->cap;
+  st.nsteps = h->d_cnt;
+  st.nrhs = h->d_cnt + c;
+  st.njac = h->d_cnt + 2 * c;
+  st.nlu = h->d_cnt + 3 * c;
+  st.nnewton = h->d_cnt + 4 * c;
+  st.nreject = h->d_cnt + 5 * c;
+  return st;
+}
+
+// Reduces the counters of the last launch and returns int(average steps) or an error.
+int finish(pp_chem* h, int ncells, hipStream_t s, float kernel_ms) {
+  ppc::CellStats st = counters(h);
+  hipLaunchKernelGGL(init_stats, dim3(1), dim3(64), 0, s, h->d_red);
+  const int blocks = ncells > 0 ? (ncells + 255) / 256 < 1024 ? (ncells + 255) / 256 : 1024 : 1;
+  hipLaunchKernelGGL(reduce_stats, dim3(blocks), dim3(256), 0, s, st.nsteps, st.nrhs, st.njac, st.nlu,
+                     st.nnewton, st.nreject, ncells, h->d_red);
+  HIPCHK(h, hipGetLastError());
+  unsigned long long r[9];
+  HIPCHK(h, hipMemcpyAsync(r, h->d_red, sizeof(r), hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  pp_chem_stats& S = h->stats;
+  S.ncells = ncells;
+  S.nsteps = (long long)r[0];
+  S.nrhs = (long long)r[1];
+  S.njac = (long long)r[2];
+  S.nlu = (long long)r[3];
+  S.nnewton = (long long)r[4];
+  S.nreject = (long long)r[5];
+  S.nfailed = (int)r[6];
+  S.max_steps_cell = (int)r[7];
+  S.min_steps_cell = (int)(r[8] == 0x7fffffffULL ? 0 : r[8]);
+  S.kernel_ms = kernel_ms;
+  if (S.nfailed > 0)
+    return fail(h, PP_CHEM_EINTEGRATE,
+                std::to_string(S.nfailed) + " cell(s) failed to integrate (NaN, step limit or step too small)");
+  if (ncells == 0) return 0;
+  return (int)((double)S.nsteps / (double)ncells);
+}
+
+ppc::ReactParamsDev to_dev(const pp_chem_config& c) {
+  ppc::ReactParamsDev P{};
+  P.reactor_type = c.reactor_type;
+  P.bdf = {c.bdf_nsubsteps, c.bdf_nonlinear_iters, c.bdf_gmres_restarts, c.bdf_gmres_kspiters,
+           c.bdf_gmres_precond, c.bdf_scheme, c.bdf_nonlinear_tol, c.bdf_gmres_tol};
+  P.rk64 = {c.rk64_atol, c.rk64_nsubsteps_guess, c.rk64_nsubsteps_min, c.rk64_nsubsteps_max};
+  P.adapt = {c.rtol, c.atol, c.first_step, c.max_steps};
+  return P;
+}
+
+ppc::A4 to_a4(const pp_array4* a) {
+  ppc::A4 r{};
+  if (!a || !a->p) return r;
+  r.p = a->p;
+  r.js = a->jstride;
+  r.ks = a->kstride;
+  r.ns = a->nstride;
+  r.bx = a->begin[0];
+  r.by = a->begin[1];
+  r.bz = a->begin[2];
+  return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pp_chem_list_mechanisms(char* buf, int buflen) {
+  std::string s;
+#define PPC_MECH(M) s += (s.empty() ? "" : ",") + std::string(#M);
+#include "inst/registry.inc"
+#undef PPC_MECH
+  if (buf && buflen > 0) {
+    std::strncpy(buf, s.c_str(), (size_t)buflen - 1);
+    buf[buflen - 1] = '\0';
+  }
+  return (int)s.size() + 1;
+}
+
+int pp_chem_config_init(pp_chem_config* cfg, const char* mechanism, int mode, int reactor_type) {
+  if (!cfg) return PP_CHEM_EINVAL;
+  const MechLauncher* L = find_mech(mechanism);
+  if (!L) return PP_CHEM_EMECH;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->mechanism = L->name;
+  cfg->reactor_type = reactor_type;
+  cfg->mode = mode;
+  cfg->mod_reactor = 0;
+  // ReactorBDF defaults (ReactorBDF.H:118-128)
+  cfg->bdf_nsubsteps = 1;
+  cfg->bdf_nonlinear_iters = 1;
+  cfg->bdf_gmres_restarts = 10;
+  cfg->bdf_gmres_kspiters = L->ns + 1;
+  cfg->bdf_gmres_precond = 2;
+  cfg->bdf_scheme = 3;
+  cfg->bdf_nonlinear_tol = 1e-6;
+  cfg->bdf_gmres_tol = 1e-10;
+  // ReactorRK64 defaults (ReactorRK64.H:131-134)
+  cfg->rk64_atol = 1e-10;
+  cfg->rk64_nsubsteps_guess = 10;
+  cfg->rk64_nsubsteps_min = 5;
+  cfg->rk64_nsubsteps_max = 500;
+  // adaptive BDF
+  cfg->rtol = 1e-8;
+  cfg->atol = 1e-10;
+  cfg->first_step = 0.0;
+  cfg->max_steps = 100000;
+  cfg->block_size = 0;
+  return PP_CHEM_OK;
+}
+
+int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
+  if (!cfg || !out) return PP_CHEM_EINVAL;
+  *out = nullptr;
+  const MechLauncher* L = find_mech(cfg->mechanism);
+  if (!L) return PP_CHEM_EMECH;
+  // ReactorTypes::check_reactor_type (ReactorTypes.H:12-19) as an error code
+  if (cfg->reactor_type != 1 && cfg->reactor_type != 2) return PP_CHEM_EINVAL;
+  if (cfg->mode < PP_CHEM_BDF || cfg->mode > PP_CHEM_BDF_ADAPTIVE_MIXED) return PP_CHEM_EINVAL;
+  if (cfg->mode == PP_CHEM_BDF &&
+      (cfg->bdf_nsubsteps < 1 || cfg->bdf_scheme < 0 || cfg->bdf_scheme > 3 ||
+       cfg->bdf_gmres_precond < 0 || cfg->bdf_gmres_precond > 2 || cfg->bdf_gmres_kspiters < 1 ||
+       cfg->bdf_gmres_kspiters > L->ns + 1 || cfg->bdf_nonlinear_iters < 1))
+    return PP_CHEM_EINVAL;
+  if (cfg->mode == PP_CHEM_RK64 &&
+      (cfg->rk64_nsubsteps_guess < 1 || cfg->rk64_nsubsteps_min < 1 || cfg->rk64_nsubsteps_max < 1))
+    return PP_CHEM_EINVAL;
+  if ((cfg->mode == PP_CHEM_BDF_ADAPTIVE || cfg->mode == PP_CHEM_BDF_ADAPTIVE_MIXED) &&
+      (!(cfg->rtol > 0) || !(cfg->atol > 0) || cfg->max_steps < 1))
+    return PP_CHEM_EINVAL;
+  const int bs = cfg->block_size > 0 ? cfg->block_size : 64;
+  if (bs > 64 || bs % 64 != 0) return PP_CHEM_EINVAL;  // kernels are built for 64-lane blocks
+  pp_chem* h = new pp_chem();
+  h->cfg = *cfg;
+  h->mech_name = L->name;
+  h->cfg.mechanism = h->mech_name.c_str();
+  h->cfg.block_size = bs;
+  h->L = L;
+  h->P = to_dev(*cfg);
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = hipMalloc(&h->d_red, sizeof(unsigned long long) * 9);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  if (e != hipSuccess) {
+    pp_chem_destroy(h);
+    return PP_CHEM_EHIP;
+  }
+  *out = h;
+  return PP_CHEM_OK;
+}
+
+void pp_chem_destroy(pp_chem* h) {
+  if (!h) return;
+  if (h->d_cnt) (void)hipFree(h->d_cnt);
+  if (h->d_red) (void)hipFree(h->d_red);
+  if (h->d_stage) (void)hipFree(h->d_stage);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  delete h;
+}
+
+int pp_chem_num_species(const pp_chem* h) { return h ? h->L->ns : PP_CHEM_EINVAL; }
+
+int pp_chem_react_box(pp_chem* h, const int lo[3], const int hi[3], const pp_array4* rY,
+                      const pp_array4* rYsrc, const pp_array4* T, const pp_array4* rE,
+                      const pp_array4* rEsrc, const pp_array4* FC, const pp_array4* mask,
+                      double* dt_react, double* time, void* stream) {
+  (void)mask;  // ignored by the reference (ReactorBDF.cpp:322)
+  if (!h) return PP_CHEM_EINVAL;
+  if (!lo || !hi || !rY || !rYsrc || !T || !rE || !rEsrc || !dt_react || !time)
+    return fail(h, PP_CHEM_EINVAL, "null argument");
+  if (!rY->p || !rYsrc->p || !T->p || !rE->p || !rEsrc->p) return fail(h, PP_CHEM_EINVAL, "null array");
+  long long nc = 1;
+  ppc::BoxArgs b{};
+  for (int d = 0; d < 3; ++d) {
+    b.lo[d] = lo[d];
+    b.len[d] = hi[d] - lo[d] + 1;
+    if (b.len[d] < 0) return fail(h, PP_CHEM_EINVAL, "hi < lo");
+    nc *= b.len[d];
+  }
+  if (nc > 0x7fffffffLL) return fail(h, PP_CHEM_EINVAL, "box too large");
+  const int ncells = (int)nc;
+  // every view must cover the box and have enough components
+  const pp_array4* views[] = {rY, rYsrc, T, rE, rEsrc, FC};
+  const int need[] = {h->L->ns, h->L->ns, 1, 1, 1, 1};
+  for (int v = 0; v < 6; ++v) {
+    const pp_array4* a = views[v];
+    if (!a || !a->p) continue;
+    for (int d = 0; d < 3; ++d)
+      if (lo[d] < a->begin[d] || hi[d] >= a->end[d]) return fail(h, PP_CHEM_EINVAL, "Array4 does not cover the box");
+    if (a->ncomp < need[v]) return fail(h, PP_CHEM_EINVAL, "Array4 has too few components");
+  }
+  b.rY = to_a4(rY);
+  b.rYsrc = to_a4(rYsrc);
+  b.T = to_a4(T);
+  b.rE = to_a4(rE);
+  b.rEsrc = to_a4(rEsrc);
+  b.FC = to_a4(FC);
+  if (int r = ensure_counters(h, ncells > 0 ? ncells : 1)) return r;
+  hipStream_t s = (hipStream_t)stream;
+  HIPCHK(h, hipEventRecord(h->ev0, s));
+  if (ncells > 0) {
+    hipError_t e = h->L->react_box(h->cfg.mode, h->P, b, *dt_react, ncells, counters(h), h->cfg.block_size, s);
+    if (e != hipSuccess) return hip_fail(h, e, "react_box launch");
+  }
+  HIPCHK(h, hipEventRecord(h->ev1, s));
+  HIPCHK(h, hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  const int r = finish(h, ncells, s, ms);
+  if (h->cfg.mod_reactor) *time += *dt_react;
+  return r;
+}
+
+int pp_chem_react_1d(pp_chem* h, double* y, double* ysrc, double* rX, double* rXsrc, double* dt_react,
+                     double* time, int ncells, int on_device, void* stream) {
+  if (!h) return PP_CHEM_EINVAL;
+  if (!dt_react || !time || ncells < 0) return fail(h, PP_CHEM_EINVAL, "bad argument");
+  if (ncells > 0 && (!y || !ysrc || !rX || !rXsrc)) return fail(h, PP_CHEM_EINVAL, "null array");
+  const int ns = h->L->ns, n = ns + 1;
+  hipStream_t s = (hipStream_t)stream;
+  if (int r = ensure_counters(h, ncells > 0 ? ncells : 1)) return r;
+  double *dy = y, *dys = ysrc, *dx = rX, *dxs = rXsrc;
+  const size_t ny = (size_t)ncells * n, nys = (size_t)ncells * ns, nx = (size_t)ncells;
+  if (!on_device && ncells > 0) {
+    const size_t need = ny + nys + 2 * nx;
+    if (need > h->stage_cap) {
+      if (h->d_stage) HIPCHK(h, hipFree(h->d_stage));
+      h->d_stage = nullptr;
+      HIPCHK(h, hipMalloc(&h->d_stage, need * sizeof(double)));
+      h->stage_cap = need;
+    }
+    dy = h->d_stage;
+    dys = dy + ny;
+    dx = dys + nys;
+    dxs = dx + nx;
+    HIPCHK(h, hipMemcpyAsync(dy, y, ny * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(dys, ysrc, nys * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(dx, rX, nx * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(dxs, rXsrc, nx * sizeof(double), hipMemcpyHostToDevice, s));
+  }
+  HIPCHK(h, hipEventRecord(h->ev0, s));
+  if (ncells > 0) {
+    hipError_t e = h->L->react_1d(h->cfg.mode, h->P, dy, dys, dx, dxs, *dt_react, ncells, counters(h),
+                                  h->cfg.block_size, s);
+    if (e != hipSuccess) return hip_fail(h, e, "react_1d launch");
+  }
+  HIPCHK(h, hipEventRecord(h->ev1, s));
+  if (!on_device && ncells > 0) {
+    HIPCHK(h, hipMemcpyAsync(y, dy, ny * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipMemcpyAsync(rX, dx, nx * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(h, hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  const int r = finish(h, ncells, s, ms);
+  if (h->cfg.mod_reactor) *time += *dt_react;
+  return r;
+}
+
+int pp_chem_get_stats(const pp_chem* h, pp_chem_stats* out) {
+  if (!h || !out) return PP_CHEM_EINVAL;
+  *out = h->stats;
+  return PP_CHEM_OK;
+}
+
+int pp_chem_copy_cell_steps(pp_chem* h, int* host_out, int ncells) {
+  if (!h || !host_out || ncells < 0 || ncells > h->cap) return PP_CHEM_EINVAL;
+  HIPCHK(h, hipMemcpy(host_out, h->d_cnt, sizeof(int) * (size_t)ncells, hipMemcpyDeviceToHost));
+  return PP_CHEM_OK;
+}
+
+const char* pp_chem_last_error(const pp_chem* h) { return h ? h->err.c_str() : "null handle"; }
+
+}  // extern "C"

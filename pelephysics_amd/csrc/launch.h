@@ -1,0 +1,62 @@
+// Per-mechanism launcher table: the C-ABI dispatches by mechanism name at run
+// time.  Kernel instantiations live in one translation unit per (mechanism,
+// integrator) pair (inst/<mech>_<mode>.hip) so they compile in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "reactors_fwd.h"
+
+namespace ppc {
+
+struct MechLauncher {
+  const char* name;
+  int ns;
+  hipError_t (*react_1d)(int mode, const ReactParamsDev& P, double* y, const double* ysrc, double* rX,
+                         const double* rXsrc, double dt, int ncells, CellStats st, int block,
+                         hipStream_t s);
+  hipError_t (*react_box)(int mode, const ReactParamsDev& P, const BoxArgs& b, double dt, int ncells,
+                          CellStats st, int block, hipStream_t s);
+};
+
+template <class M, int MODE>
+hipError_t launch_1d_mode(const ReactParamsDev& P, double* y, const double* ysrc, double* rX,
+                          const double* rXsrc, double dt, int ncells, CellStats st, int block, hipStream_t s);
+template <class M, int MODE>
+hipError_t launch_box_mode(const ReactParamsDev& P, const BoxArgs& b, double dt, int ncells, CellStats st,
+                           int block, hipStream_t s);
+
+template <class M>
+hipError_t launch_1d(int mode, const ReactParamsDev& P, double* y, const double* ysrc, double* rX,
+                     const double* rXsrc, double dt, int ncells, CellStats st, int block, hipStream_t s) {
+  switch (mode) {
+    case MODE_BDF: return launch_1d_mode<M, MODE_BDF>(P, y, ysrc, rX, rXsrc, dt, ncells, st, block, s);
+    case MODE_RK64: return launch_1d_mode<M, MODE_RK64>(P, y, ysrc, rX, rXsrc, dt, ncells, st, block, s);
+    case MODE_ADAPTIVE: return launch_1d_mode<M, MODE_ADAPTIVE>(P, y, ysrc, rX, rXsrc, dt, ncells, st, block, s);
+    case MODE_ADAPTIVE_MIXED:
+      return launch_1d_mode<M, MODE_ADAPTIVE_MIXED>(P, y, ysrc, rX, rXsrc, dt, ncells, st, block, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <class M>
+hipError_t launch_box(int mode, const ReactParamsDev& P, const BoxArgs& b, double dt, int ncells, CellStats st,
+                      int block, hipStream_t s) {
+  switch (mode) {
+    case MODE_BDF: return launch_box_mode<M, MODE_BDF>(P, b, dt, ncells, st, block, s);
+    case MODE_RK64: return launch_box_mode<M, MODE_RK64>(P, b, dt, ncells, st, block, s);
+    case MODE_ADAPTIVE: return launch_box_mode<M, MODE_ADAPTIVE>(P, b, dt, ncells, st, block, s);
+    case MODE_ADAPTIVE_MIXED: return launch_box_mode<M, MODE_ADAPTIVE_MIXED>(P, b, dt, ncells, st, block, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace ppc
+
+
+#define PPC_DEFINE_LAUNCHER(MECH)                                                              \
+  namespace ppc {                                                                              \
+  const MechLauncher* launcher_##MECH() {                                                      \
+    static const MechLauncher L = {#MECH, mech::MECH::NS, &launch_1d<mech::MECH>,              \
+                                   &launch_box<mech::MECH>};                                  \
+    return &L;                                                                                 \
+  }                                                                                            \
+  }

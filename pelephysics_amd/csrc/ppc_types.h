@@ -1,0 +1,28 @@
+// Parameter blocks passed by value to the kernels.
+#pragma once
+
+namespace ppc {
+
+struct BDFParamsDev {  // ReactorBDF ode.bdf_* (Source/Reactions/ReactorBDF.H:118-128)
+  int nsubsteps, nonlinear_iters, gmres_restarts, gmres_kspiters, gmres_precond, scheme;
+  double nonlin_tol, gmres_tol;
+};
+
+struct RK64ParamsDev {  // ReactorRK64 ode.* (Source/Reactions/ReactorRK64.H:131-134)
+  double atol;
+  int nsub_guess, nsub_min, nsub_max;
+};
+
+struct AdaptParamsDev {  // adaptive BDF tolerances
+  double rtol, atol, first_step;
+  int max_steps;
+};
+
+struct ReactParamsDev {
+  int reactor_type;
+  BDFParamsDev bdf;
+  RK64ParamsDev rk64;
+  AdaptParamsDev adapt;
+};
+
+}  // namespace ppc

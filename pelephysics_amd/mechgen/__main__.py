@@ -1,0 +1,43 @@
+"""Regenerate mechanism tables and HIP headers from the reference YAML files.
+
+    python -m pelephysics_amd.mechgen [--ref /root/reference] [mech ...]
+
+Writes ``pelephysics_amd/mechanisms/<mech>.json`` (normalised parameter
+tables, read by the oracle and the tests) and
+``pelephysics_amd/csrc/mech/<mech>.h`` (device code).
+"""
+import argparse
+import os
+
+from .model import load_yaml, Mechanism
+from .emit_hip import emit
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+DEFAULT_MECHS = ["LiDryer", "drm19"]
+
+
+def generate(mechs, ref="/root/reference"):
+    out = []
+    for name in mechs:
+        src = os.path.join(ref, "Mechanisms", name, "mechanism.yaml")
+        m = load_yaml(src, name)
+        with open(os.path.join(PKG, "mechanisms", f"{name}.json"), "w") as f:
+            f.write(m.to_json())
+        with open(os.path.join(PKG, "csrc", "mech", f"{name}.h"), "w") as f:
+            f.write(emit(m, f"Mechanisms/{name}/mechanism.yaml"))
+        out.append(name)
+    return out
+
+
+def load(name: str) -> Mechanism:
+    with open(os.path.join(PKG, "mechanisms", f"{name}.json")) as f:
+        return Mechanism.from_json(f.read())
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default="/root/reference")
+    ap.add_argument("mechs", nargs="*", default=DEFAULT_MECHS)
+    a = ap.parse_args()
+    print(generate(a.mechs, a.ref))

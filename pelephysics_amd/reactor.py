@@ -1,0 +1,281 @@
+"""Python mirror of ``pele::physics::reactions::ReactorBase`` over the C-ABI.
+
+Reference interface (Source/Reactions/ReactorBase.H:14-106, Source/Factory.H):
+
+* ``ReactorBase.create(name)`` selects an integrator by its identifier
+  ("ReactorBDF", "ReactorRK64") plus this project's adaptive forms
+  ("ReactorBDFAdaptive", "ReactorBDFAdaptiveMixed"); unknown names raise with
+  the list of valid names (``Factory.H:135-147`` aborts with the same list);
+* ``init(reactor_type, ncells)`` takes the ``ode.*`` keys as keyword
+  arguments (the ParmParse queries of ``ReactorBDF.cpp:118-128``,
+  ``ReactorRK64.cpp:12-18``) and the mechanism name;
+* ``react(box...)`` / ``react_1d(...)`` have the argument meaning of the two
+  ``react`` overloads; arrays are torch tensors on the GPU (updated in place)
+  or, for the 1-D form, host numpy arrays (copied in and out as the
+  reference's GPU build does);
+* the return value is ``int(average steps)``; errors raise ``ReactorError``
+  (the reference aborts).
+
+The computation is done by ``libpele_chem_hip.so`` (HIP kernels for gfx950);
+there is no CPU fallback: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpele_chem_hip.so")
+
+BDF, RK64, BDF_ADAPTIVE, BDF_ADAPTIVE_MIXED = 0, 1, 2, 3
+E_REACTOR, H_REACTOR = 1, 2  # Source/Reactions/ReactorTypes.H:10-11
+
+IDENTIFIERS = {
+    "ReactorBDF": BDF,
+    "ReactorRK64": RK64,
+    "ReactorBDFAdaptive": BDF_ADAPTIVE,
+    "ReactorBDFAdaptiveMixed": BDF_ADAPTIVE_MIXED,
+}
+
+ERRORS = {-1: "EINVAL", -2: "EMECH", -3: "EHIP", -4: "EINTEGRATE"}
+
+
+class ReactorError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("mechanism", C.c_char_p), ("reactor_type", C.c_int), ("mode", C.c_int),
+        ("mod_reactor", C.c_int),
+        ("bdf_nsubsteps", C.c_int), ("bdf_nonlinear_iters", C.c_int),
+        ("bdf_gmres_restarts", C.c_int), ("bdf_gmres_kspiters", C.c_int),
+        ("bdf_gmres_precond", C.c_int), ("bdf_scheme", C.c_int),
+        ("bdf_nonlinear_tol", C.c_double), ("bdf_gmres_tol", C.c_double),
+        ("rk64_atol", C.c_double), ("rk64_nsubsteps_guess", C.c_int),
+        ("rk64_nsubsteps_min", C.c_int), ("rk64_nsubsteps_max", C.c_int),
+        ("rtol", C.c_double), ("atol", C.c_double), ("first_step", C.c_double),
+        ("max_steps", C.c_int), ("block_size", C.c_int),
+    ]
+
+
+class Array4(C.Structure):
+    _fields_ = [("p", C.c_void_p), ("jstride", C.c_longlong), ("kstride", C.c_longlong),
+                ("nstride", C.c_longlong), ("begin", C.c_int * 3), ("end", C.c_int * 3),
+                ("ncomp", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("ncells", C.c_longlong), ("nsteps", C.c_longlong), ("nrhs", C.c_longlong),
+                ("njac", C.c_longlong), ("nlu", C.c_longlong), ("nnewton", C.c_longlong),
+                ("nreject", C.c_longlong), ("max_steps_cell", C.c_int),
+                ("min_steps_cell", C.c_int), ("nfailed", C.c_int), ("kernel_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# (ode.* key, config field) pairs accepted by init()
+_KEYS = {
+    "bdf_nsubsteps": "bdf_nsubsteps", "bdf_nonlinear_iters": "bdf_nonlinear_iters",
+    "bdf_gmres_restarts": "bdf_gmres_restarts", "bdf_gmres_kspiters": "bdf_gmres_kspiters",
+    "bdf_gmres_precond": "bdf_gmres_precond", "bdf_scheme": "bdf_scheme",
+    "bdf_nonlinear_tol": "bdf_nonlinear_tol", "bdf_gmres_tol": "bdf_gmres_tol",
+    "atol": None,  # RK64 absolute tolerance and adaptive atol
+    "rtol": "rtol", "first_step": "first_step", "max_steps": "max_steps",
+    "rk64_nsubsteps_guess": "rk64_nsubsteps_guess", "rk64_nsubsteps_min": "rk64_nsubsteps_min",
+    "rk64_nsubsteps_max": "rk64_nsubsteps_max", "mod_reactor": "mod_reactor",
+    "block_size": "block_size",
+}
+
+_lib = None
+
+
+def library() -> C.CDLL:
+    """Load libpele_chem_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run python -m pelephysics_amd.build")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.pp_chem_config_init.argtypes = [C.POINTER(Config), C.c_char_p, C.c_int, C.c_int]
+        L.pp_chem_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+        L.pp_chem_destroy.argtypes = [vp]
+        L.pp_chem_num_species.argtypes = [vp]
+        L.pp_chem_list_mechanisms.argtypes = [C.c_char_p, C.c_int]
+        a4 = C.POINTER(Array4)
+        L.pp_chem_react_box.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), a4, a4, a4, a4,
+                                         a4, a4, a4, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double), vp]
+        L.pp_chem_react_1d.argtypes = [vp, vp, vp, vp, vp, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double), C.c_int, C.c_int, vp]
+        L.pp_chem_get_stats.argtypes = [vp, C.POINTER(Stats)]
+        L.pp_chem_copy_cell_steps.argtypes = [vp, vp, C.c_int]
+        L.pp_chem_last_error.argtypes = [vp]
+        L.pp_chem_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def mechanisms() -> Sequence[str]:
+    L = library()
+    n = L.pp_chem_list_mechanisms(None, 0)
+    buf = C.create_string_buffer(n)
+    L.pp_chem_list_mechanisms(buf, n)
+    return buf.value.decode().split(",")
+
+
+def _array4_from_tensor(t, ncomp_axis_first=True) -> Array4:
+    """Array4 view of a contiguous torch tensor shaped (ncomp, nz, ny, nx) or (nz, ny, nx)."""
+    if t.dim() == 3:
+        t = t.unsqueeze(0)
+    if t.dim() != 4:
+        raise ValueError("expected (ncomp, nz, ny, nx)")
+    nc, nz, ny, nx = t.shape
+    st = t.stride()
+    a = Array4()
+    a.p = t.data_ptr()
+    a.nstride, a.kstride, a.jstride = st[0], st[1], st[2]
+    if st[3] != 1:
+        raise ValueError("the i index must be unit-stride")
+    a.begin[:] = [0, 0, 0]
+    a.end[:] = [nx, ny, nz]
+    a.ncomp = nc
+    return a
+
+
+class ReactorBase:
+    """Factory + common driver; subclasses fix the integrator."""
+
+    identifier = "ReactorBase"
+    _mode: int = -1
+
+    @staticmethod
+    def create(name: str) -> "ReactorBase":
+        cls = _REGISTRY.get(name)
+        if cls is None:
+            raise ReactorError(-1, f"unknown reactor {name!r}; valid: {sorted(_REGISTRY)}")
+        return cls()
+
+    def __init__(self):
+        self._h = C.c_void_p()
+        self.reactor_type = 0
+        self.mechanism = None
+
+    def init(self, reactor_type: int, ncells: int = 0, mechanism: str = "drm19", **ode) -> int:
+        """ReactorX::init(reactor_type, ncells); ode.* keys as keyword arguments."""
+        L = library()
+        cfg = Config()
+        rc = L.pp_chem_config_init(C.byref(cfg), mechanism.encode(), self._mode, reactor_type)
+        if rc != 0:
+            raise ReactorError(rc, f"mechanism {mechanism!r} not compiled; have {mechanisms()}")
+        for k, v in ode.items():
+            if k not in _KEYS:
+                raise ReactorError(-1, f"unknown ode key {k!r}")
+            if k == "atol":
+                cfg.rk64_atol = float(v)
+                cfg.atol = float(v)
+            else:
+                setattr(cfg, _KEYS[k], v)
+        self._cfg = cfg
+        self.close()
+        rc = L.pp_chem_create(C.byref(cfg), C.byref(self._h))
+        if rc != 0:
+            raise ReactorError(rc, "Wrong reactor type. Choose between 1 (e) or 2 (h)."
+                               if reactor_type not in (1, 2) else "invalid configuration")
+        self.reactor_type = reactor_type
+        self.mechanism = mechanism
+        self.nspec = L.pp_chem_num_species(self._h)
+        return 0
+
+    def close(self):
+        if self._h:
+            library().pp_chem_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> int:
+        if rc < 0:
+            raise ReactorError(rc, library().pp_chem_last_error(self._h).decode())
+        return rc
+
+    def react(self, rY, rYsrc, T, rE, rEsrc, FC, dt_react: float, time: float = 0.0,
+              mask=None, stream: Optional[int] = None):
+        """Box form on device tensors shaped (ncomp, nz, ny, nx); returns (avg_steps, time)."""
+        nz, ny, nx = T.shape[-3:]
+        lo = (C.c_int * 3)(0, 0, 0)
+        hi = (C.c_int * 3)(nx - 1, ny - 1, nz - 1)
+        views = [_array4_from_tensor(t) for t in (rY, rYsrc, T, rE, rEsrc)]
+        fc = _array4_from_tensor(FC) if FC is not None else Array4()
+        dt = C.c_double(dt_react)
+        tm = C.c_double(time)
+        rc = library().pp_chem_react_box(self._h, lo, hi, *[C.byref(v) for v in views], C.byref(fc),
+                                         None, C.byref(dt), C.byref(tm), stream)
+        return self._check(rc), tm.value
+
+    def react_1d(self, y, ysrc, rX, rXsrc, dt_react: float, time: float = 0.0,
+                 stream: Optional[int] = None):
+        """1-D form: torch device tensors (in place) or numpy host arrays (copied)."""
+        ncells = int(rX.shape[0])
+        if isinstance(y, np.ndarray):
+            for a in (y, ysrc, rX, rXsrc):
+                if a.dtype != np.float64 or not a.flags.c_contiguous:
+                    raise ValueError("host arrays must be contiguous float64")
+            ptrs = [a.ctypes.data for a in (y, ysrc, rX, rXsrc)]
+            on_dev = 0
+        else:
+            for a in (y, ysrc, rX, rXsrc):
+                if not a.is_contiguous():
+                    raise ValueError("device tensors must be contiguous")
+            ptrs = [a.data_ptr() for a in (y, ysrc, rX, rXsrc)]
+            on_dev = 1
+        dt = C.c_double(dt_react)
+        tm = C.c_double(time)
+        rc = library().pp_chem_react_1d(self._h, *ptrs, C.byref(dt), C.byref(tm), ncells, on_dev,
+                                        stream)
+        return self._check(rc), tm.value
+
+    def stats(self) -> dict:
+        s = Stats()
+        library().pp_chem_get_stats(self._h, C.byref(s))
+        return s.as_dict()
+
+    def cell_steps(self, ncells: int) -> np.ndarray:
+        out = np.zeros(ncells, np.int32)
+        self._check(library().pp_chem_copy_cell_steps(self._h, out.ctypes.data, ncells))
+        return out
+
+
+class ReactorBDF(ReactorBase):
+    identifier = "ReactorBDF"
+    _mode = BDF
+
+
+class ReactorRK64(ReactorBase):
+    identifier = "ReactorRK64"
+    _mode = RK64
+
+
+class ReactorBDFAdaptive(ReactorBase):
+    identifier = "ReactorBDFAdaptive"
+    _mode = BDF_ADAPTIVE
+
+
+class ReactorBDFAdaptiveMixed(ReactorBase):
+    identifier = "ReactorBDFAdaptiveMixed"
+    _mode = BDF_ADAPTIVE_MIXED
+
+
+_REGISTRY = {c.identifier: c for c in (ReactorBDF, ReactorRK64, ReactorBDFAdaptive,
+                                       ReactorBDFAdaptiveMixed)}

@@ -1,0 +1,154 @@
+"""Synthetic cell states for the benchmark and test workloads.
+
+Every workload returns the reactor state in the reference's CGS units as
+numpy arrays: ``rY`` (ncells, N) in g/cm^3, ``T`` (ncells,) in K, ``rE``
+(ncells,) = rho*h in erg/cm^3 (h-form, reactor_type 2) and zero sources.
+
+* ``lidryer_h2air``: H2/air, T ~ U[1000,1400] K, p = 1 atm, phi ~ U[0.5,1.5],
+  radicals (H, O, OH, HO2, H2O2) at Y = 1e-8, H2O = 0 (32^3 cells by default).
+* ``drm19_grf``: CH4/air; mixture fraction from a periodic Gaussian random
+  field (correlation length 8 cells) mapped through the normal CDF, progress
+  c ~ U[0,1] interpolating linearly between fresh gas at 300 K and complete
+  combustion products at 2200 K (64^3 cells by default).
+* ``drm19_fresh``: CH4/air, T ~ U[1100,1500] K, phi ~ U[0.6,1.4], p = 1 atm.
+
+Seeds are fixed (numpy PCG64) so every run sees the same cells.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from .mechgen.model import Mechanism, RU, PATM
+
+SEED = 20261202
+
+
+@dataclass
+class CellState:
+    mech: str
+    rY: np.ndarray  # (ncells, N)
+    T: np.ndarray  # (ncells,)
+    rE: np.ndarray  # (ncells,)
+    p: np.ndarray  # (ncells,)
+
+    @property
+    def ncells(self) -> int:
+        return self.T.shape[0]
+
+    def y_1d(self) -> np.ndarray:
+        """YCOrder vector y[icell*(N+1)+k], k = N is T."""
+        return np.ascontiguousarray(np.concatenate([self.rY, self.T[:, None]], axis=1).reshape(-1))
+
+
+def _nasa(m: Mechanism):
+    lo = np.array([s.low for s in m.species])
+    hi = np.array([s.high for s in m.species])
+    tmid = np.array([s.tmid for s in m.species])
+    return lo, hi, tmid
+
+
+def h_mass(m: Mechanism, T: np.ndarray, Y: np.ndarray) -> np.ndarray:
+    """Mixture enthalpy (erg/g) for T (ncells,), Y (ncells, N), NASA7."""
+    lo, hi, tmid = _nasa(m)
+    T = np.asarray(T, float)[:, None]
+    a = np.where(T < tmid[None, :], lo[None, :, :].transpose(2, 0, 1), hi[None, :, :].transpose(2, 0, 1))
+    hrt = a[0] + a[1] / 2 * T + a[2] / 3 * T ** 2 + a[3] / 4 * T ** 3 + a[4] / 5 * T ** 4 + a[5] / T
+    imw = np.array([s.imw for s in m.species])
+    return (Y * imw[None, :] * hrt).sum(axis=1) * RU * T[:, 0]
+
+
+def mean_mw(m: Mechanism, Y: np.ndarray) -> np.ndarray:
+    imw = np.array([s.imw for s in m.species])
+    return 1.0 / (Y * imw[None, :]).sum(axis=1)
+
+
+def _finish(m: Mechanism, name: str, Y: np.ndarray, T: np.ndarray, p: np.ndarray) -> CellState:
+    Y = Y / Y.sum(axis=1, keepdims=True)
+    rho = p * mean_mw(m, Y) / (RU * T)
+    rY = rho[:, None] * Y
+    rE = rho * h_mass(m, T, Y)
+    return CellState(name, np.ascontiguousarray(rY), np.ascontiguousarray(T),
+                     np.ascontiguousarray(rE), p)
+
+
+def _mass_from_moles(m: Mechanism, X: dict, ncells: int) -> np.ndarray:
+    mw = np.array([s.mw for s in m.species])
+    Y = np.zeros((ncells, m.n_species))
+    for name, x in X.items():
+        Y[:, m.species_index(name)] = x * mw[m.species_index(name)]
+    return Y / Y.sum(axis=1, keepdims=True)
+
+
+def lidryer_h2air(m: Mechanism, ncells: int = 32 ** 3, seed: int = SEED) -> CellState:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    T = rng.uniform(1000.0, 1400.0, ncells)
+    phi = rng.uniform(0.5, 1.5, ncells)
+    Y = _mass_from_moles(m, {"H2": 2.0 * phi, "O2": np.ones(ncells), "N2": 3.76 * np.ones(ncells)},
+                         ncells)
+    for r in ("H", "O", "OH", "HO2", "H2O2"):
+        Y[:, m.species_index(r)] = 1e-8
+    Y[:, m.species_index("H2O")] = 0.0
+    return _finish(m, m.name, Y, T, np.full(ncells, PATM))
+
+
+def _gaussian_field(n: int, corr: float, rng) -> np.ndarray:
+    """Periodic Gaussian random field on n^3, Gaussian filter sigma = corr cells, unit variance."""
+    w = rng.standard_normal((n, n, n))
+    k = np.fft.fftfreq(n)
+    kx, ky, kz = np.meshgrid(k, k, k, indexing="ij")
+    filt = np.exp(-0.5 * (2 * np.pi * corr) ** 2 * (kx ** 2 + ky ** 2 + kz ** 2))
+    f = np.real(np.fft.ifftn(np.fft.fftn(w) * filt))
+    return (f - f.mean()) / f.std()
+
+
+def _normal_cdf(x: np.ndarray) -> np.ndarray:
+    erf = np.vectorize(math.erf)
+    return 0.5 * (1.0 + erf(x / math.sqrt(2.0)))
+
+
+def drm19_grf(m: Mechanism, n: int = 64, seed: int = SEED) -> CellState:
+    """Mixture-fraction field x progress variable, cell order i fastest (x, y, z).
+
+    The correlation length is 8 cells on the 64^3 grid and scales with n on
+    smaller grids (n/8 cells)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    Z = _normal_cdf(_gaussian_field(n, 8.0 * n / 64.0, rng)).transpose(2, 1, 0).reshape(-1)  # z-major
+    ncells = Z.shape[0]
+    c = rng.uniform(0.0, 1.0, ncells)
+    mw = {s.name: s.mw for s in m.species}
+    iCH4, iO2, iN2 = (m.species_index(s) for s in ("CH4", "O2", "N2"))
+    iCO2, iH2O = m.species_index("CO2"), m.species_index("H2O")
+    Yf = np.zeros((ncells, m.n_species))
+    Yf[:, iCH4] = Z
+    Yf[:, iO2] = 0.233 * (1 - Z)
+    Yf[:, iN2] = 0.767 * (1 - Z)
+    # complete combustion CH4 + 2 O2 -> CO2 + 2 H2O of the same mixture
+    nF = Z / mw["CH4"]
+    nO = 0.233 * (1 - Z) / mw["O2"]
+    burnt = np.minimum(nF, nO / 2.0)
+    Yb = Yf.copy()
+    Yb[:, iCH4] = (nF - burnt) * mw["CH4"]
+    Yb[:, iO2] = (nO - 2 * burnt) * mw["O2"]
+    Yb[:, iCO2] = burnt * mw["CO2"]
+    Yb[:, iH2O] = 2 * burnt * mw["H2O"]
+    Y = (1 - c)[:, None] * Yf + c[:, None] * Yb
+    T = (1 - c) * 300.0 + c * 2200.0
+    return _finish(m, m.name, Y, T, np.full(ncells, PATM))
+
+
+def drm19_fresh(m: Mechanism, ncells: int = 4096, seed: int = SEED,
+                Trange=(1100.0, 1500.0), phirange=(0.6, 1.4)) -> CellState:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    T = rng.uniform(*Trange, ncells)
+    phi = rng.uniform(*phirange, ncells)
+    Y = _mass_from_moles(m, {"CH4": phi, "O2": 2.0 * np.ones(ncells),
+                             "N2": 2.0 * 3.76 * np.ones(ncells)}, ncells)
+    return _finish(m, m.name, Y, T, np.full(ncells, PATM))
+
+
+def subsample(st: CellState, idx: np.ndarray) -> CellState:
+    return CellState(st.mech, np.ascontiguousarray(st.rY[idx]), np.ascontiguousarray(st.T[idx]),
+                     np.ascontiguousarray(st.rE[idx]), st.p[idx])
