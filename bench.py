@@ -5,8 +5,9 @@
 A step is one ``react`` call (box/Array4 form, h-reactor) over the whole 64^3
 grid of the drm19 Gaussian-random-field workload, state resident in HBM; the
 pristine state is restored (device-to-device copy) before every step.  The
-headline form is the adaptive BDF with the FP32 Newton-matrix LU; the plain
-FP64 adaptive BDF is timed beside it.  Prints one JSON line.
+headline form is the batched adaptive BDF (persistent work-queue kernel, FP32
+Newton-matrix LU); the plain FP64 adaptive BDF (one lane per cell) is timed
+beside it.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -126,7 +127,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--mode", choices=["mixed", "batched-fp64", "fp64"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     a = ap.parse_args()
@@ -140,13 +141,14 @@ def main():
     st = W.drm19_grf(m, GRID)
     ncells = st.ncells
     mixed = a.mode == "mixed"
-    ident = "ReactorBDFAdaptiveMixed" if mixed else "ReactorBDFAdaptive"
+    ident = {"mixed": "ReactorBDFAdaptiveMixed", "batched-fp64": "ReactorBDFBatchedFP64",
+             "fp64": "ReactorBDFAdaptive"}[a.mode]
     log(f"[bench] {ident} drm19 {GRID}^3 = {ncells} cells, warmup {a.warmup}, steps {a.steps}")
     head = run_gpu(ident, st, m, a.steps, a.warmup, RTOL, ATOL)
     value = ncells * a.steps / head["elapsed"]
     log(f"[bench] {ident}: {value:.4g} cells/s, kernel {head['kernel_ms']:.2f} ms")
     extra = {}
-    if mixed and not a.no_plain:
+    if a.mode != "fp64" and not a.no_plain:
         plain = run_gpu("ReactorBDFAdaptive", st, m, max(1, a.steps // 2), 1, RTOL, ATOL)
         pv = ncells * max(1, a.steps // 2) / plain["elapsed"]
         Yp = plain["rY"] / plain["rY"].sum(1, keepdims=True)

@@ -1,0 +1,221 @@
+// Header-only adapter: registers the gfx950 integrators behind
+// pele::physics::reactions::ReactorBase (Source/Reactions/ReactorBase.H:14-106)
+// so PeleC / PeleLMeX select them by name, exactly like the reference's
+// ReactorBDF / ReactorRK64 (Source/Factory.H:72-79,95-124,159-162).
+//
+// Include this header in one translation unit of an AMReX + PelePhysics build
+// (it needs the reference's ReactorBase.H, AMReX and a HIP build) and link
+// libpele_chem_hip.so.  Names registered:
+//   "ReactorBDF_HIP"          -> PP_CHEM_BDF           (ReactorBDF semantics)
+//   "ReactorRK64_HIP"         -> PP_CHEM_RK64          (ReactorRK64 semantics)
+//   "ReactorBDFAdaptive_HIP"  -> PP_CHEM_BDF_ADAPTIVE  (adaptive FP64 BDF)
+//   "ReactorBDFBatched_HIP"   -> PP_CHEM_BDF_ADAPTIVE_MIXED (batched, FP32 LU)
+// The ode.* ParmParse keys of ReactorBDF::init / ReactorRK64::init are read
+// with the same names; ode.rtol / ode.atol / ode.max_steps for the adaptive
+// forms.  The mechanism name comes from ode.mechanism (default: the string of
+// the compiled PelePhysics mechanism passed as PP_CHEM_MECHANISM).
+#pragma once
+
+#include <string>
+
+#include "ReactorBase.H"
+#include "pele_chem_hip.h"
+
+namespace pele::physics::reactions {
+
+template <int MODE, const char* NAME>
+class ReactorHIP : public ReactorBase::Register<ReactorHIP<MODE, NAME>>
+{
+public:
+  static std::string identifier() { return NAME; }
+
+  int init(int reactor_type, int /*ncells*/) override
+  {
+    amrex::ParmParse pp("ode");
+    std::string mech =
+#ifdef PP_CHEM_MECHANISM
+      PP_CHEM_MECHANISM;
+#else
+      "drm19";
+#endif
+    pp.query("mechanism", mech);
+    if (pp_chem_config_init(&m_cfg, mech.c_str(), MODE, reactor_type) != 0) {
+      amrex::Abort("ReactorHIP: mechanism not compiled into libpele_chem_hip");
+    }
+    pp.query("verbose", verbose);
+    pp.query("bdf_nonlinear_iters", m_cfg.bdf_nonlinear_iters);
+    pp.query("bdf_nonlinear_tol", m_cfg.bdf_nonlinear_tol);
+    pp.query("bdf_nsubsteps", m_cfg.bdf_nsubsteps);
+    pp.query("bdf_gmres_restarts", m_cfg.bdf_gmres_restarts);
+    pp.query("bdf_gmres_kspiters", m_cfg.bdf_gmres_kspiters);
+    pp.query("bdf_gmres_tol", m_cfg.bdf_gmres_tol);
+    pp.query("bdf_gmres_precond", m_cfg.bdf_gmres_precond);
+    pp.query("bdf_scheme", m_cfg.bdf_scheme);
+    pp.query("atol", m_cfg.rk64_atol);
+    pp.query("rk64_nsubsteps_guess", m_cfg.rk64_nsubsteps_guess);
+    pp.query("rk64_nsubsteps_min", m_cfg.rk64_nsubsteps_min);
+    pp.query("rk64_nsubsteps_max", m_cfg.rk64_nsubsteps_max);
+    pp.query("rtol", m_cfg.rtol);
+    pp.query("atol", m_cfg.atol);
+    pp.query("max_steps", m_cfg.max_steps);
+#ifdef MOD_REACTOR
+    m_cfg.mod_reactor = 1;
+#endif
+    m_mech = mech;
+    m_cfg.mechanism = m_mech.c_str();
+    if (m_h) {
+      pp_chem_destroy(m_h);
+      m_h = nullptr;
+    }
+    const int rc = pp_chem_create(&m_cfg, &m_h);
+    if (rc != 0) {
+      amrex::Abort("Wrong reactor type. Choose between 1 (e) or 2 (h).");
+    }
+    return 0;
+  }
+
+  void close() override
+  {
+    if (m_h) {
+      pp_chem_destroy(m_h);
+      m_h = nullptr;
+    }
+  }
+
+  void print_final_stats(void* /*mem*/) override {}
+
+  int react(
+    const amrex::Box& box,
+    amrex::Array4<amrex::Real> const& rY_in,
+    amrex::Array4<amrex::Real> const& rYsrc_in,
+    amrex::Array4<amrex::Real> const& T_in,
+    amrex::Array4<amrex::Real> const& rEner_in,
+    amrex::Array4<amrex::Real> const& rEner_src_in,
+    amrex::Array4<amrex::Real> const& FC_in,
+    amrex::Array4<int> const& /*mask*/,
+    amrex::Real& dt_react,
+    amrex::Real& time
+#ifdef AMREX_USE_GPU
+    ,
+    amrex::gpuStream_t stream
+#endif
+    ) override
+  {
+    const auto lo = amrex::lbound(box);
+    const auto hi = amrex::ubound(box);
+    const int l[3] = {lo.x, lo.y, lo.z}, h[3] = {hi.x, hi.y, hi.z};
+    const pp_array4 a[6] = {view(rY_in), view(rYsrc_in), view(T_in),
+                            view(rEner_in), view(rEner_src_in), view(FC_in)};
+    void* s = nullptr;
+#ifdef AMREX_USE_GPU
+    s = (void*)stream;
+#endif
+    const int r = pp_chem_react_box(
+      m_h, l, h, &a[0], &a[1], &a[2], &a[3], &a[4], &a[5], nullptr, &dt_react, &time, s);
+    if (r < 0) {
+      amrex::Abort(std::string("ReactorHIP: ") + pp_chem_last_error(m_h));
+    }
+    return r;
+  }
+
+  int react(
+    amrex::Real* rY_in,
+    amrex::Real* rYsrc_in,
+    amrex::Real* rX_in,
+    amrex::Real* rX_src_in,
+    amrex::Real& dt_react,
+    amrex::Real& time,
+    int ncells
+#ifdef AMREX_USE_GPU
+    ,
+    amrex::gpuStream_t stream
+#endif
+    ) override
+  {
+    void* s = nullptr;
+#ifdef AMREX_USE_GPU
+    s = (void*)stream;
+#endif
+    // the reference's 1-D overload receives host pointers on GPU builds
+    const int r = pp_chem_react_1d(
+      m_h, rY_in, rYsrc_in, rX_in, rX_src_in, &dt_react, &time, ncells, 0, s);
+    if (r < 0) {
+      amrex::Abort(std::string("ReactorHIP: ") + pp_chem_last_error(m_h));
+    }
+    return r;
+  }
+
+  void flatten(
+    const amrex::Box& box,
+    const int ncells,
+    amrex::Array4<const amrex::Real> const& rhoY,
+    amrex::Array4<const amrex::Real> const& frcExt,
+    amrex::Array4<const amrex::Real> const& temperature,
+    amrex::Array4<const amrex::Real> const& rhoE,
+    amrex::Array4<const amrex::Real> const& frcEExt,
+    amrex::Real* y_vect,
+    amrex::Real* src_vect,
+    amrex::Real* vect_energy,
+    amrex::Real* src_vect_energy) override
+  {
+    m_flatten.flatten(
+      box, ncells, m_cfg.reactor_type, false, rhoY, frcExt, temperature, rhoE,
+      frcEExt, y_vect, src_vect, vect_energy, src_vect_energy);
+  }
+
+  void unflatten(
+    const amrex::Box& box,
+    const int ncells,
+    amrex::Array4<amrex::Real> const& rhoY,
+    amrex::Array4<amrex::Real> const& temperature,
+    amrex::Array4<amrex::Real> const& rhoE,
+    amrex::Array4<amrex::Real> const& frcEExt,
+    amrex::Array4<amrex::Real> const& FC_in,
+    amrex::Real* y_vect,
+    amrex::Real* vect_energy,
+    long int* FCunt,
+    amrex::Real dt) override
+  {
+    m_flatten.unflatten(
+      box, ncells, m_cfg.reactor_type, false, rhoY, temperature, rhoE, frcEExt,
+      FC_in, y_vect, vect_energy, FCunt, dt);
+  }
+
+  ~ReactorHIP() override { close(); }
+
+private:
+  template <class T>
+  static pp_array4 view(amrex::Array4<T> const& a)
+  {
+    pp_array4 v;
+    v.p = const_cast<double*>(reinterpret_cast<const double*>(a.p));
+    v.jstride = a.jstride;
+    v.kstride = a.kstride;
+    v.nstride = a.nstride;
+    v.begin[0] = a.begin.x;
+    v.begin[1] = a.begin.y;
+    v.begin[2] = a.begin.z;
+    v.end[0] = a.end.x;
+    v.end[1] = a.end.y;
+    v.end[2] = a.end.z;
+    v.ncomp = a.ncomp;
+    return v;
+  }
+
+  pp_chem_config m_cfg{};
+  std::string m_mech;
+  pp_chem* m_h = nullptr;
+  utils::FlattenOps<utils::YCOrder> m_flatten;
+};
+
+inline constexpr char kReactorBDF_HIP[] = "ReactorBDF_HIP";
+inline constexpr char kReactorRK64_HIP[] = "ReactorRK64_HIP";
+inline constexpr char kReactorBDFAdaptive_HIP[] = "ReactorBDFAdaptive_HIP";
+inline constexpr char kReactorBDFBatched_HIP[] = "ReactorBDFBatched_HIP";
+
+template class ReactorHIP<PP_CHEM_BDF, kReactorBDF_HIP>;
+template class ReactorHIP<PP_CHEM_RK64, kReactorRK64_HIP>;
+template class ReactorHIP<PP_CHEM_BDF_ADAPTIVE, kReactorBDFAdaptive_HIP>;
+template class ReactorHIP<PP_CHEM_BDF_ADAPTIVE_MIXED, kReactorBDFBatched_HIP>;
+
+} // namespace pele::physics::reactions

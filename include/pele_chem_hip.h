@@ -35,8 +35,10 @@ extern "C" {
 enum {
   PP_CHEM_BDF = 0,                 /* "ReactorBDF": fixed-step BDF, GMRES (parity) */
   PP_CHEM_RK64 = 1,                /* "ReactorRK64": explicit low-storage RK64 (parity) */
-  PP_CHEM_BDF_ADAPTIVE = 2,        /* adaptive-order BDF with error control, FP64 LU */
-  PP_CHEM_BDF_ADAPTIVE_MIXED = 3   /* adaptive BDF, FP32 Newton-matrix LU, FP64 elsewhere */
+  PP_CHEM_BDF_ADAPTIVE = 2,        /* adaptive-order BDF with error control, one lane per cell, FP64 */
+  PP_CHEM_BDF_ADAPTIVE_MIXED = 3,  /* batched adaptive BDF: persistent work queue, FP32 Newton-matrix
+                                      LU, FP64 residuals, rates and error control */
+  PP_CHEM_BDF_BATCHED_FP64 = 4     /* batched adaptive BDF with an FP64 LU */
 };
 
 enum {

@@ -45,11 +45,29 @@ class AdaptStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+def _cpu_has_fma() -> bool:
+    try:
+        with open("/proc/cpuinfo") as f:
+            return any(" fma " in (ln + " ") for ln in f if ln.startswith("flags"))
+    except OSError:
+        return False
+
+
 def build(force: bool = False) -> str:
-    """Compile the oracle with make (gcc); returns the library path."""
+    """Compile the oracle with make (gcc); returns the library path.
+
+    The library is built with FMA contraction when the build host has FMA; a
+    host without it rebuilds (an FMA build would not run there)."""
+    stamp = LIB + ".fma"
+    if os.path.exists(LIB) and os.path.exists(stamp) and not _cpu_has_fma():
+        force = True
     if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < max(
             os.path.getmtime(os.path.join(HERE, f)) for f in ("ppc_oracle.c", "ppc_oracle.h")):
         subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
+        if _cpu_has_fma():
+            open(stamp, "w").close()
+        elif os.path.exists(stamp):
+            os.remove(stamp)
     return LIB
 
 

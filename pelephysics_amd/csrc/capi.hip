@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -79,6 +80,54 @@ __global__ void init_stats(unsigned long long* out) {
   if (threadIdx.x < 9) out[threadIdx.x] = (threadIdx.x == 8) ? 0x7fffffffULL : 0ULL;
 }
 
+// ---- work-queue order: counting sort of the cells by the previous call's RHS
+// count, most expensive first (longest-processing-time-first scheduling).
+constexpr int NBINS = 256;
+
+__global__ void zero_ints(int* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+PPC_DI int cost_bin(int nrhs) {
+  const int b = nrhs / 2;
+  return b < 0 ? 0 : (b > NBINS - 1 ? NBINS - 1 : b);
+}
+
+__global__ void __launch_bounds__(256) cost_hist(const int* __restrict__ nrhs, int ncells, int* __restrict__ bins) {
+  __shared__ int h[NBINS];
+  for (int i = threadIdx.x; i < NBINS; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x)
+    atomicAdd(&h[cost_bin(nrhs[c])], 1);
+  __syncthreads();
+  for (int i = threadIdx.x; i < NBINS; i += blockDim.x)
+    if (h[i]) atomicAdd(&bins[i], h[i]);
+}
+
+// offs[b] = number of cells in bins above b (descending cost order); one block
+__global__ void __launch_bounds__(NBINS) cost_scan(const int* __restrict__ bins, int* __restrict__ offs) {
+  __shared__ int v[NBINS];
+  const int b = threadIdx.x;
+  v[b] = bins[NBINS - 1 - b];
+  __syncthreads();
+  for (int off = 1; off < NBINS; off <<= 1) {
+    const int add = b >= off ? v[b - off] : 0;
+    __syncthreads();
+    v[b] += add;
+    __syncthreads();
+  }
+  // v is the inclusive scan from the top bin down; exclusive offset of bin NBINS-1-b
+  offs[NBINS - 1 - b] = v[b] - bins[NBINS - 1 - b];
+}
+
+__global__ void __launch_bounds__(256) cost_scatter(const int* __restrict__ nrhs, int ncells, int* __restrict__ offs,
+                                                    int* __restrict__ order) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x) {
+    const int pos = atomicAdd(&offs[cost_bin(nrhs[c])], 1);
+    order[pos] = c;
+  }
+}
+
 }  // namespace
 
 struct pp_chem {
@@ -91,6 +140,13 @@ struct pp_chem {
   int cap = 0;
   int* d_cnt = nullptr;  // 6 * cap ints
   unsigned long long* d_red = nullptr;
+  // work queue of the batched forms: counter, visiting order, cost bins and
+  // the identity of the last call (its per-cell costs order the next call)
+  int* d_queue = nullptr;  // [0] counter, [1..NBINS] bins, [NBINS+1..2NBINS] offsets
+  int* d_order = nullptr;  // cap ints
+  bool hist_valid = false;
+  int hist_ncells = 0;
+  const void* hist_key = nullptr;
   // staging for host pointers in the 1-D form
   size_t stage_cap = 0;
   double* d_stage = nullptr;
@@ -119,8 +175,12 @@ int hip_fail(pp_chem* h, hipError_t e, const char* where) {
 int ensure_counters(pp_chem* h, int ncells) {
   if (ncells <= h->cap) return 0;
   if (h->d_cnt) HIPCHK(h, hipFree(h->d_cnt));
+  if (h->d_order) HIPCHK(h, hipFree(h->d_order));
   h->d_cnt = nullptr;
+  h->d_order = nullptr;
+  h->hist_valid = false;
   HIPCHK(h, hipMalloc(&h->d_cnt, sizeof(int) * 6 * (size_t)ncells));
+  HIPCHK(h, hipMalloc(&h->d_order, sizeof(int) * (size_t)ncells));
   h->cap = ncells;
   return 0;
 }
@@ -165,6 +225,29 @@ int finish(pp_chem* h, int ncells, hipStream_t s, float kernel_ms) {
                 std::to_string(S.nfailed) + " cell(s) failed to integrate (NaN, step limit or step too small)");
   if (ncells == 0) return 0;
   return (int)((double)S.nsteps / (double)ncells);
+}
+
+bool is_batched(int mode) { return mode == ppc::MODE_BATCHED_MIXED || mode == ppc::MODE_BATCHED_FP64; }
+
+// Prepares the work queue: zero the counter and, when the previous call
+// integrated the same cells, order them by that call's RHS counts.
+ppc::QueueArgs prepare_queue(pp_chem* h, int ncells, const void* key, hipStream_t s) {
+  ppc::QueueArgs q{nullptr, h->d_queue, ncells};
+  hipLaunchKernelGGL(zero_ints, dim3(1), dim3(256), 0, s, h->d_queue, 1 + 2 * NBINS);
+  if (h->hist_valid && h->hist_ncells == ncells && h->hist_key == key && ncells > 0) {
+    int* bins = h->d_queue + 1;
+    int* offs = bins + NBINS;
+    const int blocks = std::min(1024, (ncells + 255) / 256);
+    hipLaunchKernelGGL(cost_hist, dim3(blocks), dim3(256), 0, s, h->d_cnt + (size_t)h->cap, ncells, bins);
+    hipLaunchKernelGGL(cost_scan, dim3(1), dim3(NBINS), 0, s, bins, offs);
+    hipLaunchKernelGGL(cost_scatter, dim3(blocks), dim3(256), 0, s, h->d_cnt + (size_t)h->cap, ncells, offs,
+                       h->d_order);
+    q.order = h->d_order;
+  }
+  h->hist_valid = true;
+  h->hist_ncells = ncells;
+  h->hist_key = key;
+  return q;
 }
 
 ppc::ReactParamsDev to_dev(const pp_chem_config& c) {
@@ -245,7 +328,7 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
   if (!L) return PP_CHEM_EMECH;
   // ReactorTypes::check_reactor_type (ReactorTypes.H:12-19) as an error code
   if (cfg->reactor_type != 1 && cfg->reactor_type != 2) return PP_CHEM_EINVAL;
-  if (cfg->mode < PP_CHEM_BDF || cfg->mode > PP_CHEM_BDF_ADAPTIVE_MIXED) return PP_CHEM_EINVAL;
+  if (cfg->mode < PP_CHEM_BDF || cfg->mode > PP_CHEM_BDF_BATCHED_FP64) return PP_CHEM_EINVAL;
   if (cfg->mode == PP_CHEM_BDF &&
       (cfg->bdf_nsubsteps < 1 || cfg->bdf_scheme < 0 || cfg->bdf_scheme > 3 ||
        cfg->bdf_gmres_precond < 0 || cfg->bdf_gmres_precond > 2 || cfg->bdf_gmres_kspiters < 1 ||
@@ -254,7 +337,7 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
   if (cfg->mode == PP_CHEM_RK64 &&
       (cfg->rk64_nsubsteps_guess < 1 || cfg->rk64_nsubsteps_min < 1 || cfg->rk64_nsubsteps_max < 1))
     return PP_CHEM_EINVAL;
-  if ((cfg->mode == PP_CHEM_BDF_ADAPTIVE || cfg->mode == PP_CHEM_BDF_ADAPTIVE_MIXED) &&
+  if (cfg->mode >= PP_CHEM_BDF_ADAPTIVE &&
       (!(cfg->rtol > 0) || !(cfg->atol > 0) || cfg->max_steps < 1))
     return PP_CHEM_EINVAL;
   const int bs = cfg->block_size > 0 ? cfg->block_size : 64;
@@ -268,6 +351,7 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
   h->P = to_dev(*cfg);
   hipError_t e = hipGetDevice(&h->device);
   if (e == hipSuccess) e = hipMalloc(&h->d_red, sizeof(unsigned long long) * 9);
+  if (e == hipSuccess) e = hipMalloc(&h->d_queue, sizeof(int) * (1 + 2 * NBINS));
   if (e == hipSuccess) e = hipEventCreate(&h->ev0);
   if (e == hipSuccess) e = hipEventCreate(&h->ev1);
   if (e != hipSuccess) {
@@ -282,6 +366,8 @@ void pp_chem_destroy(pp_chem* h) {
   if (!h) return;
   if (h->d_cnt) (void)hipFree(h->d_cnt);
   if (h->d_red) (void)hipFree(h->d_red);
+  if (h->d_queue) (void)hipFree(h->d_queue);
+  if (h->d_order) (void)hipFree(h->d_order);
   if (h->d_stage) (void)hipFree(h->d_stage);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -329,7 +415,13 @@ int pp_chem_react_box(pp_chem* h, const int lo[3], const int hi[3], const pp_arr
   hipStream_t s = (hipStream_t)stream;
   HIPCHK(h, hipEventRecord(h->ev0, s));
   if (ncells > 0) {
-    hipError_t e = h->L->react_box(h->cfg.mode, h->P, b, *dt_react, ncells, counters(h), h->cfg.block_size, s);
+    hipError_t e;
+    if (is_batched(h->cfg.mode)) {
+      const ppc::QueueArgs q = prepare_queue(h, ncells, rY->p, s);
+      e = h->L->batched_box(h->cfg.mode, h->P, b, *dt_react, q, counters(h), s);
+    } else {
+      e = h->L->react_box(h->cfg.mode, h->P, b, *dt_react, ncells, counters(h), h->cfg.block_size, s);
+    }
     if (e != hipSuccess) return hip_fail(h, e, "react_box launch");
   }
   HIPCHK(h, hipEventRecord(h->ev1, s));
@@ -370,8 +462,13 @@ int pp_chem_react_1d(pp_chem* h, double* y, double* ysrc, double* rX, double* rX
   }
   HIPCHK(h, hipEventRecord(h->ev0, s));
   if (ncells > 0) {
-    hipError_t e = h->L->react_1d(h->cfg.mode, h->P, dy, dys, dx, dxs, *dt_react, ncells, counters(h),
-                                  h->cfg.block_size, s);
+    hipError_t e;
+    if (is_batched(h->cfg.mode)) {
+      const ppc::QueueArgs q = prepare_queue(h, ncells, y, s);
+      e = h->L->batched_1d(h->cfg.mode, h->P, dy, dys, dx, dxs, *dt_react, q, counters(h), s);
+    } else {
+      e = h->L->react_1d(h->cfg.mode, h->P, dy, dys, dx, dxs, *dt_react, ncells, counters(h), h->cfg.block_size, s);
+    }
     if (e != hipSuccess) return hip_fail(h, e, "react_1d launch");
   }
   HIPCHK(h, hipEventRecord(h->ev1, s));

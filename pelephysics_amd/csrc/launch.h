@@ -15,6 +15,11 @@ struct MechLauncher {
                          hipStream_t s);
   hipError_t (*react_box)(int mode, const ReactParamsDev& P, const BoxArgs& b, double dt, int ncells,
                           CellStats st, int block, hipStream_t s);
+  // batched (persistent, work-queue) forms; grid = number of 64-lane blocks
+  hipError_t (*batched_1d)(int mode, const ReactParamsDev& P, double* y, const double* ysrc, double* rX,
+                           const double* rXsrc, double dt, QueueArgs q, CellStats st, hipStream_t s);
+  hipError_t (*batched_box)(int mode, const ReactParamsDev& P, const BoxArgs& b, double dt, QueueArgs q,
+                            CellStats st, hipStream_t s);
 };
 
 template <class M, int MODE>
@@ -24,6 +29,29 @@ template <class M, int MODE>
 hipError_t launch_box_mode(const ReactParamsDev& P, const BoxArgs& b, double dt, int ncells, CellStats st,
                            int block, hipStream_t s);
 
+template <class M, int MODE>
+hipError_t batched_1d_mode(const ReactParamsDev& P, double* y, const double* ysrc, double* rX, const double* rXsrc,
+                           double dt, QueueArgs q, CellStats st, hipStream_t s);
+template <class M, int MODE>
+hipError_t batched_box_mode(const ReactParamsDev& P, const BoxArgs& b, double dt, QueueArgs q, CellStats st,
+                            hipStream_t s);
+
+template <class M>
+hipError_t launch_batched_1d(int mode, const ReactParamsDev& P, double* y, const double* ysrc, double* rX,
+                             const double* rXsrc, double dt, QueueArgs q, CellStats st, hipStream_t s) {
+  if (mode == MODE_BATCHED_MIXED) return batched_1d_mode<M, MODE_BATCHED_MIXED>(P, y, ysrc, rX, rXsrc, dt, q, st, s);
+  if (mode == MODE_BATCHED_FP64) return batched_1d_mode<M, MODE_BATCHED_FP64>(P, y, ysrc, rX, rXsrc, dt, q, st, s);
+  return hipErrorInvalidValue;
+}
+
+template <class M>
+hipError_t launch_batched_box(int mode, const ReactParamsDev& P, const BoxArgs& b, double dt, QueueArgs q,
+                              CellStats st, hipStream_t s) {
+  if (mode == MODE_BATCHED_MIXED) return batched_box_mode<M, MODE_BATCHED_MIXED>(P, b, dt, q, st, s);
+  if (mode == MODE_BATCHED_FP64) return batched_box_mode<M, MODE_BATCHED_FP64>(P, b, dt, q, st, s);
+  return hipErrorInvalidValue;
+}
+
 template <class M>
 hipError_t launch_1d(int mode, const ReactParamsDev& P, double* y, const double* ysrc, double* rX,
                      const double* rXsrc, double dt, int ncells, CellStats st, int block, hipStream_t s) {
@@ -31,8 +59,6 @@ hipError_t launch_1d(int mode, const ReactParamsDev& P, double* y, const double*
     case MODE_BDF: return launch_1d_mode<M, MODE_BDF>(P, y, ysrc, rX, rXsrc, dt, ncells, st, block, s);
     case MODE_RK64: return launch_1d_mode<M, MODE_RK64>(P, y, ysrc, rX, rXsrc, dt, ncells, st, block, s);
     case MODE_ADAPTIVE: return launch_1d_mode<M, MODE_ADAPTIVE>(P, y, ysrc, rX, rXsrc, dt, ncells, st, block, s);
-    case MODE_ADAPTIVE_MIXED:
-      return launch_1d_mode<M, MODE_ADAPTIVE_MIXED>(P, y, ysrc, rX, rXsrc, dt, ncells, st, block, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -44,7 +70,6 @@ hipError_t launch_box(int mode, const ReactParamsDev& P, const BoxArgs& b, doubl
     case MODE_BDF: return launch_box_mode<M, MODE_BDF>(P, b, dt, ncells, st, block, s);
     case MODE_RK64: return launch_box_mode<M, MODE_RK64>(P, b, dt, ncells, st, block, s);
     case MODE_ADAPTIVE: return launch_box_mode<M, MODE_ADAPTIVE>(P, b, dt, ncells, st, block, s);
-    case MODE_ADAPTIVE_MIXED: return launch_box_mode<M, MODE_ADAPTIVE_MIXED>(P, b, dt, ncells, st, block, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -56,7 +81,8 @@ hipError_t launch_box(int mode, const ReactParamsDev& P, const BoxArgs& b, doubl
   namespace ppc {                                                                              \
   const MechLauncher* launcher_##MECH() {                                                      \
     static const MechLauncher L = {#MECH, mech::MECH::NS, &launch_1d<mech::MECH>,              \
-                                   &launch_box<mech::MECH>};                                  \
+                                   &launch_box<mech::MECH>, &launch_batched_1d<mech::MECH>,   \
+                                   &launch_batched_box<mech::MECH>};                          \
     return &L;                                                                                 \
   }                                                                                            \
   }

@@ -1,8 +1,9 @@
 // Kernel launch definitions, included by the per-(mechanism, integrator)
 // instantiation units only.
 #pragma once
+#include <type_traits>
 #include "launch.h"
-#include "reactors.h"
+#include "batched.h"
 
 namespace ppc {
 
@@ -22,6 +23,44 @@ hipError_t launch_box_mode(const ReactParamsDev& P, const BoxArgs& b, double dt,
   return hipGetLastError();
 }
 
+// Persistent launch: as many 64-lane blocks as can be resident at once (never
+// more than the cell count needs); lanes take cells from the queue.
+template <class K>
+int resident_blocks(K kernel, int ncells) {
+  int dev = 0, ncu = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0) != hipSuccess) return -1;
+  if (per_cu < 1) per_cu = 1;
+  const long long need = (ncells + 63) / 64;
+  const long long blocks = (long long)ncu * per_cu;
+  return (int)(blocks < need ? blocks : need);
+}
+
+template <class M, int MODE>
+hipError_t batched_1d_mode(const ReactParamsDev& P, double* y, const double* ysrc, double* rX, const double* rXsrc,
+                           double dt, QueueArgs q, CellStats st, hipStream_t s) {
+  using LU_T = typename std::conditional<MODE == MODE_BATCHED_MIXED, float, double>::type;
+  auto k = batched_kernel<M, LU_T, IO1D>;
+  const int grid = resident_blocks(k, q.ncells);
+  if (grid < 0) return hipErrorInvalidDevice;
+  IO1D io{y, ysrc, rX, rXsrc};
+  hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
+  return hipGetLastError();
+}
+
+template <class M, int MODE>
+hipError_t batched_box_mode(const ReactParamsDev& P, const BoxArgs& b, double dt, QueueArgs q, CellStats st,
+                            hipStream_t s) {
+  using LU_T = typename std::conditional<MODE == MODE_BATCHED_MIXED, float, double>::type;
+  auto k = batched_kernel<M, LU_T, IOBox>;
+  const int grid = resident_blocks(k, q.ncells);
+  if (grid < 0) return hipErrorInvalidDevice;
+  IOBox io{b};
+  hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
+  return hipGetLastError();
+}
+
 }  // namespace ppc
 
 #define PPC_INSTANTIATE_MODE(MECH, MODE)                                                              \
@@ -31,4 +70,13 @@ hipError_t launch_box_mode(const ReactParamsDev& P, const BoxArgs& b, double dt,
                                                        int, hipStream_t);                             \
   template hipError_t launch_box_mode<mech::MECH, MODE>(const ReactParamsDev&, const BoxArgs&, double, \
                                                         int, CellStats, int, hipStream_t);            \
+  }
+
+#define PPC_INSTANTIATE_BATCHED(MECH, MODE)                                                             \
+  namespace ppc {                                                                                      \
+  template hipError_t batched_1d_mode<mech::MECH, MODE>(const ReactParamsDev&, double*, const double*,  \
+                                                        double*, const double*, double, QueueArgs,      \
+                                                        CellStats, hipStream_t);                       \
+  template hipError_t batched_box_mode<mech::MECH, MODE>(const ReactParamsDev&, const BoxArgs&, double,  \
+                                                         QueueArgs, CellStats, hipStream_t);           \
   }

@@ -5,7 +5,9 @@
 
 #define PPC_HD __host__ __device__
 #define PPC_DI __device__ __forceinline__
+#ifndef PPC_DEV
 #define PPC_DEV __device__
+#endif
 
 namespace ppc {
 

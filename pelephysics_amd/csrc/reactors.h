@@ -584,9 +584,7 @@ PPC_DI int integrate_cell(const ReactParamsDev& P, double* y, double rhoe, doubl
                           const double* src, double dt, int* cnt) {
   if (MODE == MODE_BDF) return bdf_parity_cell<M>(P.bdf, P.reactor_type, y, rhoe, rhoesrc, src, dt);
   if (MODE == MODE_RK64) return rk64_cell<M>(P.rk64, P.reactor_type, y, rhoe, rhoesrc, src, dt);
-  if (MODE == MODE_ADAPTIVE)
-    return adaptive_cell<M, double>(P.adapt, P.reactor_type, y, rhoe, rhoesrc, src, dt, cnt);
-  return adaptive_cell<M, float>(P.adapt, P.reactor_type, y, rhoe, rhoesrc, src, dt, cnt);
+  return adaptive_cell<M, double>(P.adapt, P.reactor_type, y, rhoe, rhoesrc, src, dt, cnt);
 }
 
 PPC_DI void store_counters(const CellStats& st, int c, int nsteps, const int* cnt) {

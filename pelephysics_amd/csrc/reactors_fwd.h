@@ -5,7 +5,10 @@
 
 namespace ppc {
 
-enum : int { MODE_BDF = 0, MODE_RK64 = 1, MODE_ADAPTIVE = 2, MODE_ADAPTIVE_MIXED = 3 };
+// integrators: 0 ReactorBDF parity, 1 ReactorRK64 parity, 2 adaptive BDF (plain,
+// one lane per cell, FP64), 3 batched adaptive BDF with FP32 LU, 4 batched
+// adaptive BDF with FP64 LU
+enum : int { MODE_BDF = 0, MODE_RK64 = 1, MODE_ADAPTIVE = 2, MODE_BATCHED_MIXED = 3, MODE_BATCHED_FP64 = 4 };
 
 // ------------------------------------------------------------------ I/O --
 struct A4 {  // amrex::Array4 layout
@@ -20,6 +23,12 @@ struct A4 {  // amrex::Array4 layout
 struct BoxArgs {
   int lo[3], len[3];
   A4 rY, rYsrc, T, rE, rEsrc, FC;
+};
+
+struct QueueArgs {
+  const int* order;  // cell visiting order (null: identity)
+  int* counter;      // global work counter (zeroed before the launch)
+  int ncells;
 };
 
 struct CellStats {

@@ -4,7 +4,8 @@ Reference interface (Source/Reactions/ReactorBase.H:14-106, Source/Factory.H):
 
 * ``ReactorBase.create(name)`` selects an integrator by its identifier
   ("ReactorBDF", "ReactorRK64") plus this project's adaptive forms
-  ("ReactorBDFAdaptive", "ReactorBDFAdaptiveMixed"); unknown names raise with
+  ("ReactorBDFAdaptive", "ReactorBDFAdaptiveMixed", "ReactorBDFBatchedFP64");
+  unknown names raise with
   the list of valid names (``Factory.H:135-147`` aborts with the same list);
 * ``init(reactor_type, ncells)`` takes the ``ode.*`` keys as keyword
   arguments (the ParmParse queries of ``ReactorBDF.cpp:118-128``,
@@ -30,7 +31,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpele_chem_hip.so")
 
-BDF, RK64, BDF_ADAPTIVE, BDF_ADAPTIVE_MIXED = 0, 1, 2, 3
+BDF, RK64, BDF_ADAPTIVE, BDF_ADAPTIVE_MIXED, BDF_BATCHED_FP64 = 0, 1, 2, 3, 4
 E_REACTOR, H_REACTOR = 1, 2  # Source/Reactions/ReactorTypes.H:10-11
 
 IDENTIFIERS = {
@@ -38,6 +39,7 @@ IDENTIFIERS = {
     "ReactorRK64": RK64,
     "ReactorBDFAdaptive": BDF_ADAPTIVE,
     "ReactorBDFAdaptiveMixed": BDF_ADAPTIVE_MIXED,
+    "ReactorBDFBatchedFP64": BDF_BATCHED_FP64,
 }
 
 ERRORS = {-1: "EINVAL", -2: "EMECH", -3: "EHIP", -4: "EINTEGRATE"}
@@ -273,9 +275,16 @@ class ReactorBDFAdaptive(ReactorBase):
 
 
 class ReactorBDFAdaptiveMixed(ReactorBase):
+    """Batched adaptive BDF: persistent work-queue kernel, FP32 Newton-matrix LU."""
     identifier = "ReactorBDFAdaptiveMixed"
     _mode = BDF_ADAPTIVE_MIXED
 
 
+class ReactorBDFBatchedFP64(ReactorBase):
+    """Batched adaptive BDF with an FP64 LU (isolates the FP32 LU's effect)."""
+    identifier = "ReactorBDFBatchedFP64"
+    _mode = BDF_BATCHED_FP64
+
+
 _REGISTRY = {c.identifier: c for c in (ReactorBDF, ReactorRK64, ReactorBDFAdaptive,
-                                       ReactorBDFAdaptiveMixed)}
+                                       ReactorBDFAdaptiveMixed, ReactorBDFBatchedFP64)}
