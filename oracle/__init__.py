@@ -14,6 +14,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libppc_oracle.so")
+LIB_NOFMA = os.path.join(HERE, "libppc_oracle_nofma.so")
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -61,7 +62,8 @@ def build(force: bool = False) -> str:
     stamp = LIB + ".fma"
     if os.path.exists(LIB) and os.path.exists(stamp) and not _cpu_has_fma():
         force = True
-    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < max(
+    if force or not os.path.exists(LIB) or not os.path.exists(LIB_NOFMA) or min(
+            os.path.getmtime(LIB), os.path.getmtime(LIB_NOFMA)) < max(
             os.path.getmtime(os.path.join(HERE, f)) for f in ("ppc_oracle.c", "ppc_oracle.h")):
         subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
         if _cpu_has_fma():
@@ -71,14 +73,15 @@ def build(force: bool = False) -> str:
     return LIB
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
+def lib(variant: str = "fma"):
+    """The oracle library: "fma" (default, fused multiply-add contraction where
+    the host has it) or "nofma" (separately rounded), for rounding-spread checks."""
+    if variant not in _libs:
         build()
-        L = C.CDLL(LIB)
+        L = C.CDLL(LIB if variant == "fma" else LIB_NOFMA)
         vp = C.c_void_p
         L.ppo_mech_create.restype = vp
         L.ppo_mech_create.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip,
@@ -114,8 +117,8 @@ def lib():
         L.ppo_rk64_linear.argtypes = [C.c_int, _dp, C.c_void_p, _dp, C.c_double, C.POINTER(RK64Params)]
         L.ppo_adaptive_linear.argtypes = [C.c_int, _dp, C.c_void_p, _dp, C.c_double,
                                           C.POINTER(AdaptParams), C.POINTER(AdaptStats)]
-        _lib = L
-    return _lib
+        _libs[variant] = L
+    return _libs[variant]
 
 
 def bdf_params(ns: int, **kw) -> BDFParams:
@@ -140,8 +143,9 @@ def adapt_params(rtol=1e-8, atol=1e-10, first_step=0.0, max_steps=100000) -> Ada
 class Mech:
     """Oracle view of one mechanism (tables from pelephysics_amd/mechanisms/<name>.json)."""
 
-    def __init__(self, model):
+    def __init__(self, model, variant: str = "fma"):
         self.model = model
+        self.variant = variant
         ns, nr = model.n_species, model.n_reactions
         self.ns, self.nr, self.n = ns, nr, ns + 1
         sp = model.species
@@ -178,86 +182,86 @@ class Mech:
                 nup[j, k] = v
         self._keep = [tmid, lo, hi, kind, rev, arr, arrl, troen, troe, eff, nur, nup]
         self.nu_net = nup - nur
-        self.h = lib().ppo_mech_create(ns, nr, self.mw, self.imw, tmid, lo, hi, kind, rev,
+        self.h = lib(variant).ppo_mech_create(ns, nr, self.mw, self.imw, tmid, lo, hi, kind, rev,
                                         arr.reshape(-1), arrl.reshape(-1), troen,
                                         troe.reshape(-1), eff.reshape(-1), nur.reshape(-1),
                                         nup.reshape(-1))
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            _lib.ppo_mech_destroy(self.h)
+        if getattr(self, "h", None) and self.variant in _libs:
+            _libs[self.variant].ppo_mech_destroy(self.h)
             self.h = None
 
     @staticmethod
-    def load(name: str) -> "Mech":
+    def load(name: str, variant: str = "fma") -> "Mech":
         from pelephysics_amd.mechgen.model import Mechanism
         path = os.path.join(os.path.dirname(HERE), "pelephysics_amd", "mechanisms", f"{name}.json")
         with open(path) as f:
-            return Mech(Mechanism.from_json(f.read()))
+            return Mech(Mechanism.from_json(f.read()), variant)
 
     # -- thermo
     def cp_R(self, T):
-        o = np.zeros(self.ns); lib().ppo_cp_R(self.h, T, o); return o
+        o = np.zeros(self.ns); lib(self.variant).ppo_cp_R(self.h, T, o); return o
 
     def h_RT(self, T):
-        o = np.zeros(self.ns); lib().ppo_h_RT(self.h, T, o); return o
+        o = np.zeros(self.ns); lib(self.variant).ppo_h_RT(self.h, T, o); return o
 
     def g_RT(self, T):
-        o = np.zeros(self.ns); lib().ppo_g_RT(self.h, T, o); return o
+        o = np.zeros(self.ns); lib(self.variant).ppo_g_RT(self.h, T, o); return o
 
     def h_cp_mass(self, T, Y):
         h, c = C.c_double(), C.c_double()
-        lib().ppo_hbms_cpbs(self.h, T, np.ascontiguousarray(Y, float), C.byref(h), C.byref(c))
+        lib(self.variant).ppo_hbms_cpbs(self.h, T, np.ascontiguousarray(Y, float), C.byref(h), C.byref(c))
         return h.value, c.value
 
     def e_cv_mass(self, T, Y):
         h, c = C.c_double(), C.c_double()
-        lib().ppo_ubms_cvbs(self.h, T, np.ascontiguousarray(Y, float), C.byref(h), C.byref(c))
+        lib(self.variant).ppo_ubms_cvbs(self.h, T, np.ascontiguousarray(Y, float), C.byref(h), C.byref(c))
         return h.value, c.value
 
     def T_from_h(self, h, Y, Tguess):
         e = C.c_int()
-        T = lib().ppo_T_given_hy(self.h, h, np.ascontiguousarray(Y, float), Tguess, C.byref(e))
+        T = lib(self.variant).ppo_T_given_hy(self.h, h, np.ascontiguousarray(Y, float), Tguess, C.byref(e))
         return T, e.value
 
     def T_from_e(self, ev, Y, Tguess):
         e = C.c_int()
-        T = lib().ppo_T_given_ey(self.h, ev, np.ascontiguousarray(Y, float), Tguess, C.byref(e))
+        T = lib(self.variant).ppo_T_given_ey(self.h, ev, np.ascontiguousarray(Y, float), Tguess, C.byref(e))
         return T, e.value
 
     # -- kinetics (SI-mol)
     def wdot(self, sc, T):
         o = np.zeros(self.ns)
-        lib().ppo_production_rate(self.h, np.ascontiguousarray(sc, float), T, o)
+        lib(self.variant).ppo_production_rate(self.h, np.ascontiguousarray(sc, float), T, o)
         return o
 
     def qdot(self, sc, T):
         o = np.zeros(self.nr)
-        lib().ppo_progress_rates(self.h, np.ascontiguousarray(sc, float), T, o)
+        lib(self.variant).ppo_progress_rates(self.h, np.ascontiguousarray(sc, float), T, o)
         return o
 
     def jacobian(self, sc, T, consP):
         o = np.zeros(self.n * self.n)
-        lib().ppo_jacobian(self.h, np.ascontiguousarray(sc, float), T, consP, o)
+        lib(self.variant).ppo_jacobian(self.h, np.ascontiguousarray(sc, float), T, consP, o)
         return o
 
     def set_dlnA(self, dlnA=None):
         if dlnA is None:
-            lib().ppo_mech_set_dlnA(self.h, None)
+            lib(self.variant).ppo_mech_set_dlnA(self.h, None)
         else:
             self._dlnA = np.ascontiguousarray(dlnA, float)
-            lib().ppo_mech_set_dlnA(self.h, self._dlnA.ctypes.data)
+            lib(self.variant).ppo_mech_set_dlnA(self.h, self._dlnA.ctypes.data)
 
     # -- reactor
     def fkernelspec(self, y, rhoe_init, rhoesrc, rYsrc, reactor_type=2, dt_save=0.0):
         o = np.zeros(self.n)
-        lib().ppo_fkernelspec(self.h, dt_save, reactor_type, np.ascontiguousarray(y, float), o,
+        lib(self.variant).ppo_fkernelspec(self.h, dt_save, reactor_type, np.ascontiguousarray(y, float), o,
                               rhoe_init, rhoesrc, np.ascontiguousarray(rYsrc, float))
         return o
 
     def chem_jac(self, y, reactor_type=2):
         o = np.zeros(self.n * self.n)
-        lib().ppo_chem_jac(self.h, reactor_type, np.ascontiguousarray(y, float), o)
+        lib(self.variant).ppo_chem_jac(self.h, reactor_type, np.ascontiguousarray(y, float), o)
         return o.reshape(self.n, self.n)
 
     def react_1d(self, mode, y, ysrc, rX, rXsrc, dt, reactor_type=2, params=None, nthreads=0,
@@ -267,18 +271,18 @@ class Mech:
         if mode == "bdf":
             p = params or bdf_params(self.ns)
             fc = np.zeros(ncells, np.int32)
-            r = lib().ppo_react_bdf_1d(self.h, C.byref(p), reactor_type, y, ysrc, rX, rXsrc, dt,
+            r = lib(self.variant).ppo_react_bdf_1d(self.h, C.byref(p), reactor_type, y, ysrc, rX, rXsrc, dt,
                                        time, ncells, fc.ctypes.data, nthreads)
             return r, fc
         if mode == "rk64":
             p = params or rk64_params()
             fc = np.zeros(ncells, np.int32)
-            r = lib().ppo_react_rk64_1d(self.h, C.byref(p), reactor_type, y, ysrc, rX, rXsrc, dt,
+            r = lib(self.variant).ppo_react_rk64_1d(self.h, C.byref(p), reactor_type, y, ysrc, rX, rXsrc, dt,
                                         time, ncells, fc.ctypes.data, nthreads)
             return r, fc
         p = params or adapt_params()
         st = AdaptStats()
-        r = lib().ppo_react_adaptive_1d(self.h, C.byref(p), reactor_type, y, ysrc, rX, rXsrc, dt,
+        r = lib(self.variant).ppo_react_adaptive_1d(self.h, C.byref(p), reactor_type, y, ysrc, rX, rXsrc, dt,
                                         time, ncells, C.byref(st), nthreads)
         return r, st.as_dict()
 
@@ -292,7 +296,7 @@ class Mech:
         pa = params if mode == "adaptive" and params is not None else adapt_params()
         st = AdaptStats()
         md = {"bdf": 0, "rk64": 1, "adaptive": 2}[mode]
-        r = lib().ppo_react_box(self.h, md, C.byref(pb), C.byref(pr), C.byref(pa), reactor_type,
+        r = lib(self.variant).ppo_react_box(self.h, md, C.byref(pb), C.byref(pr), C.byref(pa), reactor_type,
                                 rY.reshape(-1), rYsrc.reshape(-1), T, rE, rEsrc, FC, dt, time,
                                 ncells, C.byref(st), nthreads)
         return r, FC

@@ -25,7 +25,7 @@ PPC_DI double rho_and_Y(const double* __restrict__ rY, double* __restrict__ Y, d
 // applying the step) when |dT| < 1e-6 K, at most 200 iterations; linear
 // extrapolation outside [90, 4000] K.
 template <class M, bool ENTHALPY>
-PPC_DI double T_from_energy(const double hv, const double* __restrict__ Y, const double tguess) {
+PPC_HEAVY double T_from_energy(const double hv, const double* __restrict__ Y, const double tguess) {
   const double tmin = 90.0, tmax = 4000.0;
   double hmin, hmax, c;
   if (ENTHALPY) {
@@ -88,7 +88,7 @@ PPC_DI void wdot_mass(const double rho, const double T, const double* __restrict
 // T from the energy (guess = y[NS]), species rates plus external sources, and
 // the temperature equation with cv (e-form) or cp (h-form).
 template <class M>
-PPC_DI void f_kernel_spec(const double dt_save, const int reactor_type, const double* __restrict__ y,
+PPC_HEAVY void f_kernel_spec(const double dt_save, const int reactor_type, const double* __restrict__ y,
                           double* __restrict__ ydot, const double rhoe_init, const double rhoesrc_ext,
                           const double* __restrict__ rYsrc) {
   double Y[M::NS];
@@ -127,7 +127,7 @@ PPC_DI void f_kernel_spec(const double dt_save, const int reactor_type, const do
 // scaled by mw_i/mw_j, T column by mw_i, T row by 1/mw_j).  Jc is a
 // scratch array of n*n doubles; A may alias Jc.
 template <class M>
-PPC_DI void bdf_jacobian(const int reactor_type, const double* __restrict__ y, const double s,
+PPC_HEAVY void bdf_jacobian(const int reactor_type, const double* __restrict__ y, const double s,
                          double* Jc, double* A) {
   constexpr int NS = M::NS, n = NS + 1;
   double sc[NS];

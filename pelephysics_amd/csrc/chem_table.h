@@ -1,0 +1,439 @@
+// Table-driven mechanism evaluation for one lane = one cell.
+//
+// TabMech<T> exposes the interface of the generated unrolled mechanism
+// structs (mech/<name>.h) -- species thermo, mixture h/cp, production rates,
+// progress rates and the analytic Jacobian with the constant-pressure
+// convention -- but walks the constant-memory tables of tab/<name>.h with a
+// loop index that is the same for every lane of the wavefront: parameters
+// arrive through scalar loads, the loop body is shared by all reactions of a
+// group, and the instruction footprint stays small enough for the
+// instruction cache (the unrolled form of drm19 is ~0.5 MB of code).
+//
+// Equilibrium constants use per-species exponentials: with a_k = exp(-g_k/RT)
+// and b_k = c_k exp(g_k/RT), the reverse rate of reaction j is
+// k_f refC^delta prod_reactants a_k prod_products b_k, the same quantity as
+// k_f exp(-(sum nu' g - sum nu'' g)) refC^delta prod c'' of the reference
+// (Mechanisms/drm19/mechanism.H:4421-5667) with NS instead of NR exponentials;
+// partial products stay below e^470 for 200 K <= T <= 4000 K.
+#pragma once
+#include "ppc_common.h"
+
+namespace ppc {
+
+template <class T>
+struct TabMech {
+  static constexpr int NS = T::NS, NR = T::NR, NEQ = T::NEQ;
+  static constexpr bool kTable = true;
+
+  PPC_DI static double mw(int k) { return T::mw_(k); }
+  PPC_DI static double imw(int k) { return T::imw_(k); }
+  PPC_DI static double mw_rt(int k) { return T::mw_(k); }
+  PPC_DI static double imw_rt(int k) { return T::imw_(k); }
+
+  // NASA7 coefficients of species k for temperature Tv (per-lane range select)
+  PPC_DI static void nasa(int k, double Tv, double* a) {
+    const bool lo = Tv < T::tmid_(k);
+#pragma unroll
+    for (int i = 0; i < 7; ++i) a[i] = lo ? T::nasaLo_(7 * k + i) : T::nasaHi_(7 * k + i);
+  }
+  PPC_DI static double cp_poly(const double* a, double Tv) {
+    return a[0] + Tv * (a[1] + Tv * (a[2] + Tv * (a[3] + Tv * a[4])));
+  }
+  PPC_DI static double h_poly(const double* a, double Tv, double invT) {
+    return a[0] + Tv * (a[1] * 0.5 + Tv * (a[2] * (1.0 / 3.0) + Tv * (a[3] * 0.25 + Tv * a[4] * 0.2))) +
+           a[5] * invT;
+  }
+  PPC_DI static double g_poly(const double* a, double Tv, double invT, double logT) {
+    return a[5] * invT + (a[0] - a[6]) - a[0] * logT -
+           Tv * (a[1] * 0.5 + Tv * (a[2] * (1.0 / 6.0) + Tv * (a[3] * (1.0 / 12.0) + Tv * a[4] * 0.05)));
+  }
+
+  PPC_DI static void cp_R(double* out, double Tv) {
+#pragma unroll 1
+    for (int k = 0; k < NS; ++k) {
+      double a[7];
+      nasa(k, Tv, a);
+      out[k] = cp_poly(a, Tv);
+    }
+  }
+  PPC_DI static void cv_R(double* out, double Tv) {
+    cp_R(out, Tv);
+    for (int k = 0; k < NS; ++k) out[k] -= 1.0;
+  }
+  PPC_DI static void h_RT(double* out, double Tv) {
+    const double invT = 1.0 / Tv;
+#pragma unroll 1
+    for (int k = 0; k < NS; ++k) {
+      double a[7];
+      nasa(k, Tv, a);
+      out[k] = h_poly(a, Tv, invT);
+    }
+  }
+  PPC_DI static void e_RT(double* out, double Tv) {
+    h_RT(out, Tv);
+    for (int k = 0; k < NS; ++k) out[k] -= 1.0;
+  }
+  PPC_DI static void gibbs_RT(double* out, double Tv) {
+    const double invT = 1.0 / Tv, logT = log(Tv);
+#pragma unroll 1
+    for (int k = 0; k < NS; ++k) {
+      double a[7];
+      nasa(k, Tv, a);
+      out[k] = g_poly(a, Tv, invT, logT);
+    }
+  }
+  PPC_DI static void dcpR_dT(double* out, double Tv) {
+#pragma unroll 1
+    for (int k = 0; k < NS; ++k) {
+      double a[7];
+      nasa(k, Tv, a);
+      out[k] = a[1] + Tv * (2.0 * a[2] + Tv * (3.0 * a[3] + Tv * 4.0 * a[4]));
+    }
+  }
+  // mixture h (erg/g) and cp (erg/g/K), or e and cv, from mass fractions
+  PPC_DI static void h_cp_mass(double Tv, const double* Y, double& hm, double& cm) {
+    const double invT = 1.0 / Tv;
+    double hs = 0.0, cs = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < NS; ++k) {
+      double a[7];
+      nasa(k, Tv, a);
+      const double yw = Y[k] * T::imw_(k);
+      hs += yw * h_poly(a, Tv, invT);
+      cs += yw * cp_poly(a, Tv);
+    }
+    hm = hs * (RU * Tv);
+    cm = cs * RU;
+  }
+  PPC_DI static void e_cv_mass(double Tv, const double* Y, double& em, double& cm) {
+    const double invT = 1.0 / Tv;
+    double es = 0.0, cs = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < NS; ++k) {
+      double a[7];
+      nasa(k, Tv, a);
+      const double yw = Y[k] * T::imw_(k);
+      es += yw * (h_poly(a, Tv, invT) - 1.0);
+      cs += yw * (cp_poly(a, Tv) - 1.0);
+    }
+    em = es * (RU * Tv);
+    cm = cs * RU;
+  }
+
+  // refC^delta for the small integer powers that occur
+  PPC_DI static double refc_pow(int d, double refC, double refCinv) {
+    double r = 1.0;
+    for (int i = 0; i < d; ++i) r *= refC;
+    for (int i = 0; i < -d; ++i) r *= refCinv;
+    return r;
+  }
+
+  // Net rate of progress of reaction j (table order) given c (sc), a = exp(-g),
+  // b = c exp(g); returns q and (optionally) the pieces the Jacobian needs.
+  template <int KIND>  // 0 elementary, 1 three-body, 2 Troe falloff, 3 Lindemann falloff
+  PPC_DI static double rate(int j, int fo, const double* sc, const double* a, const double* b, double mixture,
+                            double Tv, double invT, double logT, double refC, double refCinv) {
+    const double kf0 = T::rxA_(j) * exp(T::rxB_(j) * logT - T::rxE_(j) * invT);
+    double corr = 1.0;
+    if (KIND >= 1) {
+      double alpha = mixture;
+      const int e0 = T::rxEff0_(j), en = T::rxEffN_(j);
+#pragma unroll 1
+      for (int i = 0; i < en; ++i) alpha += T::effV_(e0 + i) * sc[T::effK_(e0 + i)];
+      if (KIND == 1) {
+        corr = alpha;
+      } else {
+        const double k0 = T::foA_(fo) * exp(T::foB_(fo) * logT - T::foE_(fo) * invT);
+        const double redP = alpha / kf0 * k0;
+        const double F = redP / (1.0 + redP);
+        if (KIND == 2) {
+          const double tA = T::troe_(4 * fo), t3 = T::troe_(4 * fo + 1), t1 = T::troe_(4 * fo + 2);
+          double Fcent = (1.0 - tA) * exp(-Tv / t3) + tA * exp(-Tv / t1);
+          if (T::troeT2_(fo)) Fcent += exp(-T::troe_(4 * fo + 3) * invT);
+          const double logFcent = log10(Fcent);
+          const double logPred = log10(redP);
+          const double tc = -0.4 - 0.67 * logFcent;
+          const double tn = 0.75 - 1.27 * logFcent;
+          const double tr = (tc + logPred) / (tn - 0.14 * (tc + logPred));
+          corr = F * exp(LN10 * logFcent / (1.0 + tr * tr));
+        } else {
+          corr = F;
+        }
+      }
+    }
+    const double kf = corr * kf0;
+    double qf = kf;
+    double qr = 0.0;
+    const int r0 = T::rxR_(3 * j), r1 = T::rxR_(3 * j + 1), r2 = T::rxR_(3 * j + 2);
+    qf *= sc[r0];
+    if (r1 >= 0) qf *= sc[r1];
+    if (r2 >= 0) qf *= sc[r2];
+    if (T::rxRev_(j)) {
+      const int p0 = T::rxP_(3 * j), p1 = T::rxP_(3 * j + 1), p2 = T::rxP_(3 * j + 2);
+      qr = kf * refc_pow(T::rxDelta_(j), refC, refCinv) * a[r0] * b[p0];
+      if (r1 >= 0) qr *= a[r1];
+      if (r2 >= 0) qr *= a[r2];
+      if (p1 >= 0) qr *= b[p1];
+      if (p2 >= 0) qr *= b[p2];
+    }
+    return qf - qr;
+  }
+
+  // a_k = exp(-g_k/RT), b_k = c_k exp(g_k/RT), mixture = sum c
+  PPC_DI static double species_exp(const double* sc, double Tv, double* a, double* b) {
+    const double invT = 1.0 / Tv, logT = log(Tv);
+    double mixture = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < NS; ++k) {
+      double c[7];
+      nasa(k, Tv, c);
+      const double eg = exp(g_poly(c, Tv, invT, logT));
+      a[k] = 1.0 / eg;
+      b[k] = sc[k] * eg;
+      mixture += sc[k];
+    }
+    return mixture;
+  }
+
+  template <bool WDOT>
+  PPC_DI static void rates(double* __restrict__ out, const double* __restrict__ sc, const double Tv) {
+    double a[NS], b[NS];
+    const double mixture = species_exp(sc, Tv, a, b);
+    const double invT = 1.0 / Tv, logT = log(Tv);
+    const double refC = 101325.0 / 8.31446 * invT, refCinv = 1.0 / refC;
+    if (WDOT)
+      for (int k = 0; k < NS; ++k) out[k] = 0.0;
+    auto put = [&](int j, double q) {
+      if (WDOT) {
+        for (int i = 0; i < T::MAXNET; ++i) {
+          const int k = T::rxNet_(T::MAXNET * j + i);
+          if (k < 0) break;
+          out[k] += T::rxNu_(T::MAXNET * j + i) * q;
+        }
+      } else {
+        out[T::rxIndex_(j)] = q;
+      }
+    };
+    int j = 0;
+#pragma unroll 1
+    for (; j < T::N_EL; ++j) put(j, rate<0>(j, 0, sc, a, b, mixture, Tv, invT, logT, refC, refCinv));
+#pragma unroll 1
+    for (; j < T::N_EL + T::N_TB; ++j) put(j, rate<1>(j, 0, sc, a, b, mixture, Tv, invT, logT, refC, refCinv));
+#pragma unroll 1
+    for (int f = 0; f < T::N_TROE; ++f, ++j)
+      put(j, rate<2>(j, f, sc, a, b, mixture, Tv, invT, logT, refC, refCinv));
+#pragma unroll 1
+    for (int f = T::N_TROE; f < T::N_TROE + T::N_LIND; ++f, ++j)
+      put(j, rate<3>(j, f, sc, a, b, mixture, Tv, invT, logT, refC, refCinv));
+  }
+
+  // molar production rates (mol/m^3/s) from concentrations (mol/m^3)
+  PPC_DI static void production_rate(double* wdot, const double* sc, double Tv) { rates<true>(wdot, sc, Tv); }
+  // net rates of progress in YAML order
+  PPC_DI static void progress_rates(double* q, const double* sc, double Tv) { rates<false>(q, sc, Tv); }
+
+  // Contribution of reaction j (table order, falloff index fo) to the
+  // analytic Jacobian: addJ(idx, v) adds v to J[idx] (column-major,
+  // idx = k*(NS+1)+m for d wdot_m / d c_k, NS*(NS+1)+m for d wdot_m / dT) and
+  // addW(m, v) to wdot_m.  g and hRT are the species g/RT and h/RT at Tv.
+  template <int CONSP, class AddJ, class AddW>
+  PPC_DI static void jac_reaction(int j, const double* __restrict__ sc, const double* __restrict__ g,
+                                  const double* __restrict__ hRT, double mixture, double Tv, double invT,
+                                  double invT2, double logT, double refC, double refCinv, AddJ&& addJ,
+                                  AddW&& addW) {
+    constexpr int n = NS + 1;
+    const int kind = j < T::N_EL ? 0 : (j < T::N_EL + T::N_TB ? 1 : (j < NR - T::N_LIND ? 2 : 3));
+    const int fo = j - T::N_EL - T::N_TB;
+    const int r0 = T::rxR_(3 * j), r1 = T::rxR_(3 * j + 1), r2 = T::rxR_(3 * j + 2);
+    const int p0 = T::rxP_(3 * j), p1 = T::rxP_(3 * j + 1), p2 = T::rxP_(3 * j + 2);
+    const bool rev = T::rxRev_(j);
+    double phi_f = sc[r0];
+    if (r1 >= 0) phi_f *= sc[r1];
+    if (r2 >= 0) phi_f *= sc[r2];
+    double k_f = T::rxA_(j) * exp(T::rxB_(j) * logT - T::rxE_(j) * invT);
+    const double dlnkfdT = T::rxB_(j) * invT + T::rxE_(j) * invT2;
+    double alpha = 0.0, Pr = 0.0, fPr = 1.0, F = 1.0, dlogFdlogPr = 0.0, dlogFdT = 0.0, dlogfPrdT = 0.0;
+    if (kind >= 1) {
+      alpha = mixture;
+      const int e0 = T::rxEff0_(j), en = T::rxEffN_(j);
+#pragma unroll 1
+      for (int i = 0; i < en; ++i) alpha += T::effV_(e0 + i) * sc[T::effK_(e0 + i)];
+    }
+    if (kind >= 2) {
+      const double k0 = T::foA_(fo) * exp(T::foB_(fo) * logT - T::foE_(fo) * invT);
+      Pr = alpha / k_f * k0;
+      fPr = Pr / (1.0 + Pr);
+      const double dlnk0dT = T::foB_(fo) * invT + T::foE_(fo) * invT2;
+      const double dlogPrdT = LOG10E * (dlnk0dT - dlnkfdT);
+      dlogfPrdT = dlogPrdT / (1.0 + Pr);
+      if (kind == 2) {
+        const double tA = T::troe_(4 * fo), t3 = T::troe_(4 * fo + 1), t1 = T::troe_(4 * fo + 2);
+        const double t2 = T::troe_(4 * fo + 3);
+        const bool has2 = T::troeT2_(fo);
+        const double logPr = log10(Pr);
+        const double F1 = (1.0 - tA) * exp(-Tv / t3);
+        const double F2 = tA * exp(-Tv / t1);
+        const double F3 = has2 ? exp(-t2 * invT) : 0.0;
+        const double Fcent = F1 + F2 + F3;
+        const double logFcent = log10(Fcent);
+        const double tc = -0.4 - 0.67 * logFcent, tn = 0.75 - 1.27 * logFcent;
+        const double den = 1.0 / (tn - 0.14 * (tc + logPr));
+        const double tPr = (tc + logPr) * den;
+        const double troe = 1.0 / (1.0 + tPr * tPr);
+        F = exp(LN10 * logFcent * troe);
+        const double dlogFcentdT = LOG10E / Fcent * (-F1 / t3 - F2 / t1 + (has2 ? F3 * t2 * invT2 : 0.0));
+        const double fac = 2.0 * logFcent * troe * troe * tPr * den;
+        const double dlogFdc = -tn * fac * den;
+        const double dlogFdn = fac * tPr;
+        dlogFdlogPr = dlogFdc;
+        dlogFdT = dlogFcentdT * (troe - 0.67 * dlogFdc - 1.27 * dlogFdn) + dlogFdlogPr * dlogPrdT;
+      }
+    }
+    double phi_r = 0.0, k_r = 0.0, dkrdT = 0.0;
+    if (rev) {
+      phi_r = sc[p0];
+      if (p1 >= 0) phi_r *= sc[p1];
+      if (p2 >= 0) phi_r *= sc[p2];
+      double dg = g[r0] - g[p0], hf = hRT[r0], hr = hRT[p0];
+      if (r1 >= 0) { dg += g[r1]; hf += hRT[r1]; }
+      if (r2 >= 0) { dg += g[r2]; hf += hRT[r2]; }
+      if (p1 >= 0) { dg -= g[p1]; hr += hRT[p1]; }
+      if (p2 >= 0) { dg -= g[p2]; hr += hRT[p2]; }
+      const int d = T::rxDelta_(j);
+      k_r = k_f * exp(-dg) * refc_pow(d, refC, refCinv);
+      const double dlnKcdT = invT * (-hf + hr + (double)d);
+      dkrdT = (dlnkfdT - dlnKcdT) * k_r;
+    }
+    double q, dqdT, fac = 0.0;
+    if (kind == 0) {
+      q = k_f * phi_f - k_r * phi_r;
+      dqdT = dlnkfdT * k_f * phi_f - dkrdT * phi_r;
+    } else if (kind == 1) {
+      const double qn = k_f * phi_f - k_r * phi_r;
+      q = alpha * qn;
+      dqdT = alpha * (dlnkfdT * k_f * phi_f - dkrdT * phi_r);
+      k_f *= alpha;
+      k_r *= alpha;
+      fac = qn;
+    } else {
+      const double qn = k_f * phi_f - k_r * phi_r;
+      const double Corr = fPr * F;
+      q = Corr * qn;
+      const double dlnCorrdT = LN10 * (dlogfPrdT + dlogFdT);
+      dqdT = Corr * (dlnkfdT * k_f * phi_f - dkrdT * phi_r) + dlnCorrdT * q;
+      k_f *= Corr;
+      k_r *= Corr;
+      fac = q / alpha * (1.0 / (Pr + 1.0) + dlogFdlogPr);
+    }
+    // net stoichiometry of this reaction
+    int nk[T::MAXNET];
+    double nv[T::MAXNET];
+    int nn = 0;
+    for (int i = 0; i < T::MAXNET; ++i) {
+      nk[i] = T::rxNet_(T::MAXNET * j + i);
+      nv[i] = T::rxNu_(T::MAXNET * j + i);
+      if (nk[i] >= 0) nn = i + 1;
+    }
+    for (int i = 0; i < nn; ++i) {
+      addW(nk[i], nv[i] * q);
+      addJ(NS * n + nk[i], nv[i] * dqdT);
+    }
+    auto col = [&](int k, double dq) {
+      for (int i = 0; i < nn; ++i) addJ(k * n + nk[i], nv[i] * dq);
+    };
+    // d(prod c^nu)/dc_k over the expanded species lists (repeated index = coefficient 2, 3)
+    const int rl[3] = {r0, r1, r2}, pl[3] = {p0, p1, p2};
+    for (int i = 0; i < 3; ++i) {
+      const int k = rl[i];
+      if (k < 0 || (i > 0 && k == rl[i - 1])) continue;
+      int mult = 0;
+      double rest = 1.0;
+      for (int l = 0; l < 3; ++l) {
+        if (rl[l] < 0) continue;
+        if (rl[l] == k) {
+          if (mult > 0) rest *= sc[k];
+          ++mult;
+        } else {
+          rest *= sc[rl[l]];
+        }
+      }
+      col(k, k_f * (double)mult * rest);
+    }
+    if (rev) {
+      for (int i = 0; i < 3; ++i) {
+        const int k = pl[i];
+        if (k < 0 || (i > 0 && k == pl[i - 1])) continue;
+        int mult = 0;
+        double rest = 1.0;
+        for (int l = 0; l < 3; ++l) {
+          if (pl[l] < 0) continue;
+          if (pl[l] == k) {
+            if (mult > 0) rest *= sc[k];
+            ++mult;
+          } else {
+            rest *= sc[pl[l]];
+          }
+        }
+        col(k, -k_r * (double)mult * rest);
+      }
+    }
+    if (kind >= 1) {
+      if (CONSP == 0) {
+#pragma unroll 1
+        for (int k = 0; k < NS; ++k) col(k, fac);
+      }
+      const int e0 = T::rxEff0_(j), en = T::rxEffN_(j);
+#pragma unroll 1
+      for (int i = 0; i < en; ++i) col(T::effK_(e0 + i), T::effV_(e0 + i) * fac);
+    }
+  }
+
+  // Analytic Jacobian in SI-mol units, column-major J[k*(NS+1)+m] = d wdot_m / d c_k,
+  // J[NS*(NS+1)+m] = d wdot_m / dT, T row d Tdot / d c_k, d Tdot / dT
+  // (aJacobian semantics, Mechanisms/LiDryer/mechanism.H:3905-5138).
+  template <int CONSP>
+  PPC_DI static void jacobian(double* __restrict__ J, const double* __restrict__ sc, const double Tv) {
+    constexpr int n = NS + 1;
+#pragma unroll 1
+    for (int i = 0; i < n * n; ++i) J[i] = 0.0;
+    double wdot[NS], hRT[NS], g[NS];
+    for (int k = 0; k < NS; ++k) wdot[k] = 0.0;
+    const double invT = 1.0 / Tv, invT2 = invT * invT, logT = log(Tv);
+    const double refC = 101325.0 / 8.31446 * invT, refCinv = 1.0 / refC;
+    double mixture = 0.0;
+    for (int k = 0; k < NS; ++k) mixture += sc[k];
+    gibbs_RT(g, Tv);
+    h_RT(hRT, Tv);
+#pragma unroll 1
+    for (int j = 0; j < NR; ++j)
+      jac_reaction<CONSP>(j, sc, g, hRT, mixture, Tv, invT, invT2, logT, refC, refCinv,
+                          [&](int idx, double v) { J[idx] += v; }, [&](int m, double v) { wdot[m] += v; });
+    // temperature row (mechanism.H:5098-5137)
+    double cR[NS], dcR[NS];
+    cp_R(cR, Tv);
+    if (CONSP != 1)
+      for (int k = 0; k < NS; ++k) cR[k] -= 1.0;
+    dcpR_dT(dcR, Tv);
+    double cmix = 0.0, ehmix = 0.0, dcmixdT = 0.0, dehmixdT = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < NS; ++k) {
+      const double eh = (CONSP == 1) ? hRT[k] : hRT[k] - 1.0;
+      cmix += cR[k] * sc[k];
+      dcmixdT += dcR[k] * sc[k];
+      ehmix += eh * wdot[k];
+      dehmixdT += invT * (cR[k] - eh) * wdot[k] + eh * J[NS * n + k];
+    }
+    const double cmixinv = 1.0 / cmix;
+    const double tmp1 = ehmix * cmixinv, tmp3 = cmixinv * Tv, tmp2 = tmp1 * tmp3;
+#pragma unroll 1
+    for (int k = 0; k < NS; ++k) {
+      double d = 0.0;
+#pragma unroll 1
+      for (int mm = 0; mm < NS; ++mm) d += ((CONSP == 1) ? hRT[mm] : hRT[mm] - 1.0) * J[k * n + mm];
+      J[k * n + NS] = tmp2 * cR[k] - tmp3 * d;
+    }
+    J[n * n - 1] = -tmp1 + tmp2 * dcmixdT - tmp3 * dehmixdT;
+  }
+};
+
+}  // namespace ppc

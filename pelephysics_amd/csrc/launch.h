@@ -77,12 +77,11 @@ hipError_t launch_box(int mode, const ReactParamsDev& P, const BoxArgs& b, doubl
 }  // namespace ppc
 
 
-#define PPC_DEFINE_LAUNCHER(MECH)                                                              \
+#define PPC_DEFINE_LAUNCHER(NAME, TYPE)                                                        \
   namespace ppc {                                                                              \
-  const MechLauncher* launcher_##MECH() {                                                      \
-    static const MechLauncher L = {#MECH, mech::MECH::NS, &launch_1d<mech::MECH>,              \
-                                   &launch_box<mech::MECH>, &launch_batched_1d<mech::MECH>,   \
-                                   &launch_batched_box<mech::MECH>};                          \
+  const MechLauncher* launcher_##NAME() {                                                      \
+    static const MechLauncher L = {#NAME, TYPE::NS, &launch_1d<TYPE>, &launch_box<TYPE>,       \
+                                   &launch_batched_1d<TYPE>, &launch_batched_box<TYPE>};      \
     return &L;                                                                                 \
   }                                                                                            \
   }

@@ -4,8 +4,20 @@
 #include <type_traits>
 #include "launch.h"
 #include "batched.h"
+#include "wave_bdf.h"
 
 namespace ppc {
+
+// Table-driven mechanisms run the batched forms one wave per cell
+// (wave_bdf.h); generated unrolled mechanisms one lane per cell (batched.h).
+template <class M>
+struct wave_tables {
+  using type = void;
+};
+template <class T>
+struct wave_tables<TabMech<T>> {
+  using type = T;
+};
 
 template <class M, int MODE>
 hipError_t launch_1d_mode(const ReactParamsDev& P, double* y, const double* ysrc, double* rX,
@@ -24,15 +36,15 @@ hipError_t launch_box_mode(const ReactParamsDev& P, const BoxArgs& b, double dt,
 }
 
 // Persistent launch: as many 64-lane blocks as can be resident at once (never
-// more than the cell count needs); lanes take cells from the queue.
+// more than `lanes` / 64 needs); lanes or waves take cells from the queue.
 template <class K>
-int resident_blocks(K kernel, int ncells) {
+int resident_blocks(K kernel, long long lanes) {
   int dev = 0, ncu = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return -1;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0) != hipSuccess) return -1;
   if (per_cu < 1) per_cu = 1;
-  const long long need = (ncells + 63) / 64;
+  const long long need = (lanes + 63) / 64;
   const long long blocks = (long long)ncu * per_cu;
   return (int)(blocks < need ? blocks : need);
 }
@@ -41,11 +53,20 @@ template <class M, int MODE>
 hipError_t batched_1d_mode(const ReactParamsDev& P, double* y, const double* ysrc, double* rX, const double* rXsrc,
                            double dt, QueueArgs q, CellStats st, hipStream_t s) {
   using LU_T = typename std::conditional<MODE == MODE_BATCHED_MIXED, float, double>::type;
-  auto k = batched_kernel<M, LU_T, IO1D>;
-  const int grid = resident_blocks(k, q.ncells);
-  if (grid < 0) return hipErrorInvalidDevice;
-  IO1D io{y, ysrc, rX, rXsrc};
-  hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
+  using Tab = typename wave_tables<M>::type;
+  if constexpr (!std::is_void<Tab>::value) {
+    auto k = wave_bdf_kernel<Tab, LU_T, WaveIO1D>;
+    const int grid = resident_blocks(k, q.ncells * 64);
+    if (grid < 0) return hipErrorInvalidDevice;
+    WaveIO1D io{y, ysrc, rX, rXsrc};
+    hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
+  } else {
+    auto k = batched_kernel<M, LU_T, IO1D>;
+    const int grid = resident_blocks(k, q.ncells);
+    if (grid < 0) return hipErrorInvalidDevice;
+    IO1D io{y, ysrc, rX, rXsrc};
+    hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
+  }
   return hipGetLastError();
 }
 
@@ -53,30 +74,39 @@ template <class M, int MODE>
 hipError_t batched_box_mode(const ReactParamsDev& P, const BoxArgs& b, double dt, QueueArgs q, CellStats st,
                             hipStream_t s) {
   using LU_T = typename std::conditional<MODE == MODE_BATCHED_MIXED, float, double>::type;
-  auto k = batched_kernel<M, LU_T, IOBox>;
-  const int grid = resident_blocks(k, q.ncells);
-  if (grid < 0) return hipErrorInvalidDevice;
-  IOBox io{b};
-  hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
+  using Tab = typename wave_tables<M>::type;
+  if constexpr (!std::is_void<Tab>::value) {
+    auto k = wave_bdf_kernel<Tab, LU_T, WaveIOBox>;
+    const int grid = resident_blocks(k, q.ncells * 64);
+    if (grid < 0) return hipErrorInvalidDevice;
+    WaveIOBox io{b};
+    hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
+  } else {
+    auto k = batched_kernel<M, LU_T, IOBox>;
+    const int grid = resident_blocks(k, q.ncells);
+    if (grid < 0) return hipErrorInvalidDevice;
+    IOBox io{b};
+    hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
+  }
   return hipGetLastError();
 }
 
 }  // namespace ppc
 
-#define PPC_INSTANTIATE_MODE(MECH, MODE)                                                              \
+#define PPC_INSTANTIATE_MODE(TYPE, MODE)                                                              \
   namespace ppc {                                                                                    \
-  template hipError_t launch_1d_mode<mech::MECH, MODE>(const ReactParamsDev&, double*, const double*, \
+  template hipError_t launch_1d_mode<TYPE, MODE>(const ReactParamsDev&, double*, const double*, \
                                                        double*, const double*, double, int, CellStats, \
                                                        int, hipStream_t);                             \
-  template hipError_t launch_box_mode<mech::MECH, MODE>(const ReactParamsDev&, const BoxArgs&, double, \
+  template hipError_t launch_box_mode<TYPE, MODE>(const ReactParamsDev&, const BoxArgs&, double, \
                                                         int, CellStats, int, hipStream_t);            \
   }
 
-#define PPC_INSTANTIATE_BATCHED(MECH, MODE)                                                             \
+#define PPC_INSTANTIATE_BATCHED(TYPE, MODE)                                                             \
   namespace ppc {                                                                                      \
-  template hipError_t batched_1d_mode<mech::MECH, MODE>(const ReactParamsDev&, double*, const double*,  \
+  template hipError_t batched_1d_mode<TYPE, MODE>(const ReactParamsDev&, double*, const double*,  \
                                                         double*, const double*, double, QueueArgs,      \
                                                         CellStats, hipStream_t);                       \
-  template hipError_t batched_box_mode<mech::MECH, MODE>(const ReactParamsDev&, const BoxArgs&, double,  \
+  template hipError_t batched_box_mode<TYPE, MODE>(const ReactParamsDev&, const BoxArgs&, double,  \
                                                          QueueArgs, CellStats, hipStream_t);           \
   }

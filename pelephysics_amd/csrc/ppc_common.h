@@ -4,9 +4,17 @@
 #include <math.h>
 
 #define PPC_HD __host__ __device__
+#ifndef PPC_DI
 #define PPC_DI __device__ __forceinline__
+#endif
 #ifndef PPC_DEV
 #define PPC_DEV __device__
+#endif
+// Large per-cell routines (RHS, Jacobian, LU, T inversion): kept as real
+// calls so that each appears once in a kernel's code, whatever the number of
+// call sites in the integrator state machines.
+#ifndef PPC_HEAVY
+#define PPC_HEAVY __device__ __forceinline__
 #endif
 
 namespace ppc {

@@ -298,7 +298,7 @@ PPC_DI double rms_norm(const double* __restrict__ x, const double* __restrict__ 
 
 // In-place LU with partial pivoting (row-major).
 template <int n, typename T>
-PPC_DI void lu_factor(T* __restrict__ a, int* __restrict__ piv) {
+PPC_HEAVY void lu_factor(T* __restrict__ a, int* __restrict__ piv) {
   for (int k = 0; k < n; ++k) {
     int p = k;
     T mx = fabs(a[k * n + k]);
@@ -326,7 +326,7 @@ PPC_DI void lu_factor(T* __restrict__ a, int* __restrict__ piv) {
 }
 
 template <int n, typename T>
-PPC_DI void lu_solve(const T* __restrict__ a, const int* __restrict__ piv, double* __restrict__ b) {
+PPC_HEAVY void lu_solve(const T* __restrict__ a, const int* __restrict__ piv, double* __restrict__ b) {
   for (int k = 0; k < n; ++k)
     if (piv[k] != k) {
       const double t = b[k];
@@ -347,7 +347,7 @@ PPC_DI void lu_solve(const T* __restrict__ a, const int* __restrict__ piv, doubl
 
 // D[0..order] <- (R U)^T D[0..order] for a step-size ratio `factor`.
 template <int n>
-PPC_DI void change_D(double* __restrict__ D, const int order, const double factor) {
+PPC_HEAVY void change_D(double* __restrict__ D, const int order, const double factor) {
   double R[AMAXORD + 1][AMAXORD + 1], U[AMAXORD + 1][AMAXORD + 1];
   for (int pass = 0; pass < 2; ++pass) {
     const double f = pass == 0 ? factor : 1.0;

@@ -4,13 +4,15 @@
 
 Writes ``pelephysics_amd/mechanisms/<mech>.json`` (normalised parameter
 tables, read by the oracle and the tests) and
-``pelephysics_amd/csrc/mech/<mech>.h`` (device code).
+``pelephysics_amd/csrc/mech/<mech>.h`` (unrolled device code) and
+``pelephysics_amd/csrc/tab/<mech>.h`` (constant-memory tables).
 """
 import argparse
 import os
 
 from .model import load_yaml, Mechanism
 from .emit_hip import emit
+from .emit_tables import emit as emit_tables
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
@@ -26,6 +28,8 @@ def generate(mechs, ref="/root/reference"):
             f.write(m.to_json())
         with open(os.path.join(PKG, "csrc", "mech", f"{name}.h"), "w") as f:
             f.write(emit(m, f"Mechanisms/{name}/mechanism.yaml"))
+        with open(os.path.join(PKG, "csrc", "tab", f"{name}.h"), "w") as f:
+            f.write(emit_tables(m, f"Mechanisms/{name}/mechanism.yaml"))
         out.append(name)
     return out
 
