@@ -706,7 +706,14 @@ static void bdf_ws_init(bdf_ws* w, int n) {
 }
 static void bdf_ws_free(bdf_ws* w) { free(w->soln); }
 
-/* Fixed-step BDF core of ReactorBDF::react (ReactorBDF.cpp:187-285): y (n) in/out. */
+static int lu_factor(int n, double* a, int* piv);
+static void lu_solve(int n, const double* a, const int* piv, double* b);
+
+/* Fixed-step BDF core of ReactorBDF::react (ReactorBDF.cpp:187-285): y (n) in/out.
+ * gmres_precond < 0 (not a reference option) replaces the GMRES solve by an
+ * LU solve with partial pivoting: the discrete map the adjoint differentiates
+ * (the reference GMRES stops at an absolute residual of gmres_tol, which makes
+ * the map non-differentiable at that level for minor species). */
 static int bdf_core(sys_t* s, const ppo_bdf_params* p, double* y, double dt_react, bdf_ws* w) {
   const int n = s->n;
   const double dt = dt_react / (double)p->nsubsteps;
@@ -741,9 +748,16 @@ static int bdf_core(sys_t* s, const ppo_bdf_params* p, double* y, double dt_reac
         r += FCOEFF[ts][1] * w->fn[i];
         w->rhs[i] = r;
       }
-      if (gmres(n, w->A, w->rhs, w->ds0, w->ds, p->gmres_precond, p->gmres_restarts,
-                p->gmres_kspiters, p->gmres_tol, w->gw) != 0)
+      if (p->gmres_precond < 0) {
+        /* A is overwritten by its factors; gw holds n ints of pivots */
+        int* piv = (int*)w->gw;
+        memcpy(w->ds, w->rhs, sizeof(double) * n);
+        if (lu_factor(n, w->A, piv) != 0) return -1;
+        lu_solve(n, w->A, piv, w->ds);
+      } else if (gmres(n, w->A, w->rhs, w->ds0, w->ds, p->gmres_precond, p->gmres_restarts,
+                       p->gmres_kspiters, p->gmres_tol, w->gw) != 0) {
         return -1;
+      }
       for (int i = 0; i < n; ++i) w->soln[i] += w->ds[i];
       double nrm = 0.0;
       for (int i = 0; i < n; ++i) nrm += w->rhs[i] * w->rhs[i];

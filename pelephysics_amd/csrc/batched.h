@@ -78,8 +78,16 @@ struct IOBox {
 
 enum : int { PH_F0 = 0, PH_F1 = 1, PH_NEWTON = 2 };
 
+// Waves per SIMD the register allocation of the batched kernel is held to:
+// at 2 (<= 256 VGPRs) the spills grow but the second wave hides the private-
+// memory latency that dominates this kernel; drm19 64^3: 616 ms vs 780 ms at
+// the compiler's default (1 wave, 256 VGPR + AGPRs) and 654 ms at 4.
+#ifndef PPC_LANE_OCC
+#define PPC_LANE_OCC __attribute__((amdgpu_waves_per_eu(2)))
+#endif
+
 template <class M, typename LU_T, class IO>
-__global__ void __launch_bounds__(64) batched_kernel(ReactParamsDev P, IO io, double dt, QueueArgs q,
+__global__ void __launch_bounds__(64) PPC_LANE_OCC batched_kernel(ReactParamsDev P, IO io, double dt, QueueArgs q,
                                                      CellStats st) {
   constexpr int n = M::NEQ, NS = M::NS;
   const AdaptParamsDev& ap = P.adapt;

@@ -329,6 +329,7 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
   // ReactorTypes::check_reactor_type (ReactorTypes.H:12-19) as an error code
   if (cfg->reactor_type != 1 && cfg->reactor_type != 2) return PP_CHEM_EINVAL;
   if (cfg->mode < PP_CHEM_BDF || cfg->mode > PP_CHEM_BDF_BATCHED_FP64) return PP_CHEM_EINVAL;
+  if (is_batched(cfg->mode) && !L->batched_ok) return PP_CHEM_EINVAL;  // form not compiled for L
   if (cfg->mode == PP_CHEM_BDF &&
       (cfg->bdf_nsubsteps < 1 || cfg->bdf_scheme < 0 || cfg->bdf_scheme > 3 ||
        cfg->bdf_gmres_precond < 0 || cfg->bdf_gmres_precond > 2 || cfg->bdf_gmres_kspiters < 1 ||

@@ -17,6 +17,7 @@
 // partial products stay below e^470 for 200 K <= T <= 4000 K.
 #pragma once
 #include "ppc_common.h"
+#include "dual.h"
 
 namespace ppc {
 
@@ -36,16 +37,25 @@ struct TabMech {
 #pragma unroll
     for (int i = 0; i < 7; ++i) a[i] = lo ? T::nasaLo_(7 * k + i) : T::nasaHi_(7 * k + i);
   }
-  PPC_DI static double cp_poly(const double* a, double Tv) {
+  // S: double, or Dual for derivatives along T
+  template <class S>
+  PPC_DI static S cp_poly(const double* a, S Tv) {
     return a[0] + Tv * (a[1] + Tv * (a[2] + Tv * (a[3] + Tv * a[4])));
   }
-  PPC_DI static double h_poly(const double* a, double Tv, double invT) {
+  template <class S>
+  PPC_DI static S h_poly(const double* a, S Tv, S invT) {
     return a[0] + Tv * (a[1] * 0.5 + Tv * (a[2] * (1.0 / 3.0) + Tv * (a[3] * 0.25 + Tv * a[4] * 0.2))) +
            a[5] * invT;
   }
-  PPC_DI static double g_poly(const double* a, double Tv, double invT, double logT) {
+  template <class S>
+  PPC_DI static S g_poly(const double* a, S Tv, S invT, S logT) {
     return a[5] * invT + (a[0] - a[6]) - a[0] * logT -
            Tv * (a[1] * 0.5 + Tv * (a[2] * (1.0 / 6.0) + Tv * (a[3] * (1.0 / 12.0) + Tv * a[4] * 0.05)));
+  }
+  // d(cp/R)/dT
+  template <class S>
+  PPC_DI static S dcp_poly(const double* a, S Tv) {
+    return a[1] + Tv * (2.0 * a[2] + Tv * (3.0 * a[3] + Tv * 4.0 * a[4]));
   }
 
   PPC_DI static void cp_R(double* out, double Tv) {
@@ -121,8 +131,9 @@ struct TabMech {
   }
 
   // refC^delta for the small integer powers that occur
-  PPC_DI static double refc_pow(int d, double refC, double refCinv) {
-    double r = 1.0;
+  template <class S>
+  PPC_DI static S refc_pow(int d, S refC, S refCinv) {
+    S r = 1.0;
     for (int i = 0; i < d; ++i) r *= refC;
     for (int i = 0; i < -d; ++i) r *= refCinv;
     return r;
@@ -236,23 +247,23 @@ struct TabMech {
   // analytic Jacobian: addJ(idx, v) adds v to J[idx] (column-major,
   // idx = k*(NS+1)+m for d wdot_m / d c_k, NS*(NS+1)+m for d wdot_m / dT) and
   // addW(m, v) to wdot_m.  g and hRT are the species g/RT and h/RT at Tv.
-  template <int CONSP, class AddJ, class AddW>
-  PPC_DI static void jac_reaction(int j, const double* __restrict__ sc, const double* __restrict__ g,
-                                  const double* __restrict__ hRT, double mixture, double Tv, double invT,
-                                  double invT2, double logT, double refC, double refCinv, AddJ&& addJ,
-                                  AddW&& addW) {
+  // S is double, or Dual for directional derivatives of the whole contribution.
+  template <int CONSP, class S, class AddJ, class AddW>
+  PPC_DI static void jac_reaction(int j, const S* __restrict__ sc, const S* __restrict__ g,
+                                  const S* __restrict__ hRT, S mixture, S Tv, S invT, S invT2, S logT, S refC,
+                                  S refCinv, AddJ&& addJ, AddW&& addW) {
     constexpr int n = NS + 1;
     const int kind = j < T::N_EL ? 0 : (j < T::N_EL + T::N_TB ? 1 : (j < NR - T::N_LIND ? 2 : 3));
     const int fo = j - T::N_EL - T::N_TB;
     const int r0 = T::rxR_(3 * j), r1 = T::rxR_(3 * j + 1), r2 = T::rxR_(3 * j + 2);
     const int p0 = T::rxP_(3 * j), p1 = T::rxP_(3 * j + 1), p2 = T::rxP_(3 * j + 2);
     const bool rev = T::rxRev_(j);
-    double phi_f = sc[r0];
+    S phi_f = sc[r0];
     if (r1 >= 0) phi_f *= sc[r1];
     if (r2 >= 0) phi_f *= sc[r2];
-    double k_f = T::rxA_(j) * exp(T::rxB_(j) * logT - T::rxE_(j) * invT);
-    const double dlnkfdT = T::rxB_(j) * invT + T::rxE_(j) * invT2;
-    double alpha = 0.0, Pr = 0.0, fPr = 1.0, F = 1.0, dlogFdlogPr = 0.0, dlogFdT = 0.0, dlogfPrdT = 0.0;
+    S k_f = T::rxA_(j) * exp(T::rxB_(j) * logT - T::rxE_(j) * invT);
+    const S dlnkfdT = T::rxB_(j) * invT + T::rxE_(j) * invT2;
+    S alpha = 0.0, Pr = 0.0, fPr = 1.0, F = 1.0, dlogFdlogPr = 0.0, dlogFdT = 0.0, dlogfPrdT = 0.0;
     if (kind >= 1) {
       alpha = mixture;
       const int e0 = T::rxEff0_(j), en = T::rxEffN_(j);
@@ -260,66 +271,66 @@ struct TabMech {
       for (int i = 0; i < en; ++i) alpha += T::effV_(e0 + i) * sc[T::effK_(e0 + i)];
     }
     if (kind >= 2) {
-      const double k0 = T::foA_(fo) * exp(T::foB_(fo) * logT - T::foE_(fo) * invT);
+      const S k0 = T::foA_(fo) * exp(T::foB_(fo) * logT - T::foE_(fo) * invT);
       Pr = alpha / k_f * k0;
       fPr = Pr / (1.0 + Pr);
-      const double dlnk0dT = T::foB_(fo) * invT + T::foE_(fo) * invT2;
-      const double dlogPrdT = LOG10E * (dlnk0dT - dlnkfdT);
+      const S dlnk0dT = T::foB_(fo) * invT + T::foE_(fo) * invT2;
+      const S dlogPrdT = LOG10E * (dlnk0dT - dlnkfdT);
       dlogfPrdT = dlogPrdT / (1.0 + Pr);
       if (kind == 2) {
         const double tA = T::troe_(4 * fo), t3 = T::troe_(4 * fo + 1), t1 = T::troe_(4 * fo + 2);
         const double t2 = T::troe_(4 * fo + 3);
         const bool has2 = T::troeT2_(fo);
-        const double logPr = log10(Pr);
-        const double F1 = (1.0 - tA) * exp(-Tv / t3);
-        const double F2 = tA * exp(-Tv / t1);
-        const double F3 = has2 ? exp(-t2 * invT) : 0.0;
-        const double Fcent = F1 + F2 + F3;
-        const double logFcent = log10(Fcent);
-        const double tc = -0.4 - 0.67 * logFcent, tn = 0.75 - 1.27 * logFcent;
-        const double den = 1.0 / (tn - 0.14 * (tc + logPr));
-        const double tPr = (tc + logPr) * den;
-        const double troe = 1.0 / (1.0 + tPr * tPr);
+        const S logPr = log10(Pr);
+        const S F1 = (1.0 - tA) * exp(-Tv / t3);
+        const S F2 = tA * exp(-Tv / t1);
+        const S F3 = has2 ? exp(-t2 * invT) : S(0.0);
+        const S Fcent = F1 + F2 + F3;
+        const S logFcent = log10(Fcent);
+        const S tc = -0.4 - 0.67 * logFcent, tn = 0.75 - 1.27 * logFcent;
+        const S den = 1.0 / (tn - 0.14 * (tc + logPr));
+        const S tPr = (tc + logPr) * den;
+        const S troe = 1.0 / (1.0 + tPr * tPr);
         F = exp(LN10 * logFcent * troe);
-        const double dlogFcentdT = LOG10E / Fcent * (-F1 / t3 - F2 / t1 + (has2 ? F3 * t2 * invT2 : 0.0));
-        const double fac = 2.0 * logFcent * troe * troe * tPr * den;
-        const double dlogFdc = -tn * fac * den;
-        const double dlogFdn = fac * tPr;
+        const S dlogFcentdT = LOG10E / Fcent * (-F1 / t3 - F2 / t1 + (has2 ? F3 * t2 * invT2 : S(0.0)));
+        const S fac = 2.0 * logFcent * troe * troe * tPr * den;
+        const S dlogFdc = -tn * fac * den;
+        const S dlogFdn = fac * tPr;
         dlogFdlogPr = dlogFdc;
         dlogFdT = dlogFcentdT * (troe - 0.67 * dlogFdc - 1.27 * dlogFdn) + dlogFdlogPr * dlogPrdT;
       }
     }
-    double phi_r = 0.0, k_r = 0.0, dkrdT = 0.0;
+    S phi_r = 0.0, k_r = 0.0, dkrdT = 0.0;
     if (rev) {
       phi_r = sc[p0];
       if (p1 >= 0) phi_r *= sc[p1];
       if (p2 >= 0) phi_r *= sc[p2];
-      double dg = g[r0] - g[p0], hf = hRT[r0], hr = hRT[p0];
+      S dg = g[r0] - g[p0], hf = hRT[r0], hr = hRT[p0];
       if (r1 >= 0) { dg += g[r1]; hf += hRT[r1]; }
       if (r2 >= 0) { dg += g[r2]; hf += hRT[r2]; }
       if (p1 >= 0) { dg -= g[p1]; hr += hRT[p1]; }
       if (p2 >= 0) { dg -= g[p2]; hr += hRT[p2]; }
       const int d = T::rxDelta_(j);
       k_r = k_f * exp(-dg) * refc_pow(d, refC, refCinv);
-      const double dlnKcdT = invT * (-hf + hr + (double)d);
+      const S dlnKcdT = invT * (-hf + hr + (double)d);
       dkrdT = (dlnkfdT - dlnKcdT) * k_r;
     }
-    double q, dqdT, fac = 0.0;
+    S q, dqdT, fac = 0.0;
     if (kind == 0) {
       q = k_f * phi_f - k_r * phi_r;
       dqdT = dlnkfdT * k_f * phi_f - dkrdT * phi_r;
     } else if (kind == 1) {
-      const double qn = k_f * phi_f - k_r * phi_r;
+      const S qn = k_f * phi_f - k_r * phi_r;
       q = alpha * qn;
       dqdT = alpha * (dlnkfdT * k_f * phi_f - dkrdT * phi_r);
       k_f *= alpha;
       k_r *= alpha;
       fac = qn;
     } else {
-      const double qn = k_f * phi_f - k_r * phi_r;
-      const double Corr = fPr * F;
+      const S qn = k_f * phi_f - k_r * phi_r;
+      const S Corr = fPr * F;
       q = Corr * qn;
-      const double dlnCorrdT = LN10 * (dlogfPrdT + dlogFdT);
+      const S dlnCorrdT = LN10 * (dlogfPrdT + dlogFdT);
       dqdT = Corr * (dlnkfdT * k_f * phi_f - dkrdT * phi_r) + dlnCorrdT * q;
       k_f *= Corr;
       k_r *= Corr;
@@ -338,7 +349,7 @@ struct TabMech {
       addW(nk[i], nv[i] * q);
       addJ(NS * n + nk[i], nv[i] * dqdT);
     }
-    auto col = [&](int k, double dq) {
+    auto col = [&](int k, const S& dq) {
       for (int i = 0; i < nn; ++i) addJ(k * n + nk[i], nv[i] * dq);
     };
     // d(prod c^nu)/dc_k over the expanded species lists (repeated index = coefficient 2, 3)
@@ -347,7 +358,7 @@ struct TabMech {
       const int k = rl[i];
       if (k < 0 || (i > 0 && k == rl[i - 1])) continue;
       int mult = 0;
-      double rest = 1.0;
+      S rest = 1.0;
       for (int l = 0; l < 3; ++l) {
         if (rl[l] < 0) continue;
         if (rl[l] == k) {
@@ -364,7 +375,7 @@ struct TabMech {
         const int k = pl[i];
         if (k < 0 || (i > 0 && k == pl[i - 1])) continue;
         int mult = 0;
-        double rest = 1.0;
+        S rest = 1.0;
         for (int l = 0; l < 3; ++l) {
           if (pl[l] < 0) continue;
           if (pl[l] == k) {

@@ -3,13 +3,26 @@
 // integrator) pair (inst/<mech>_<mode>.hip) so they compile in parallel.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "reactors_fwd.h"
 
 namespace ppc {
 
+// The batched forms of table-driven mechanisms run one wave per cell
+// (wave_bdf.h); that form is compiled only with PPC_WAVE_BATCHED=1.
+#ifndef PPC_WAVE_BATCHED
+#define PPC_WAVE_BATCHED 0
+#endif
+
+template <class M, class = void>
+struct is_table_mech : std::false_type {};
+template <class M>
+struct is_table_mech<M, std::void_t<decltype(M::kTable)>> : std::true_type {};
+
 struct MechLauncher {
   const char* name;
   int ns;
+  int batched_ok;  // the batched modes are available
   hipError_t (*react_1d)(int mode, const ReactParamsDev& P, double* y, const double* ysrc, double* rX,
                          const double* rXsrc, double dt, int ncells, CellStats st, int block,
                          hipStream_t s);
@@ -80,7 +93,9 @@ hipError_t launch_box(int mode, const ReactParamsDev& P, const BoxArgs& b, doubl
 #define PPC_DEFINE_LAUNCHER(NAME, TYPE)                                                        \
   namespace ppc {                                                                              \
   const MechLauncher* launcher_##NAME() {                                                      \
-    static const MechLauncher L = {#NAME, TYPE::NS, &launch_1d<TYPE>, &launch_box<TYPE>,       \
+    static const MechLauncher L = {#NAME, TYPE::NS,                                            \
+                                   !is_table_mech<TYPE>::value || PPC_WAVE_BATCHED,            \
+                                   &launch_1d<TYPE>, &launch_box<TYPE>,                        \
                                    &launch_batched_1d<TYPE>, &launch_batched_box<TYPE>};      \
     return &L;                                                                                 \
   }                                                                                            \

@@ -55,11 +55,15 @@ hipError_t batched_1d_mode(const ReactParamsDev& P, double* y, const double* ysr
   using LU_T = typename std::conditional<MODE == MODE_BATCHED_MIXED, float, double>::type;
   using Tab = typename wave_tables<M>::type;
   if constexpr (!std::is_void<Tab>::value) {
+#if PPC_WAVE_BATCHED
     auto k = wave_bdf_kernel<Tab, LU_T, WaveIO1D>;
     const int grid = resident_blocks(k, q.ncells * 64);
     if (grid < 0) return hipErrorInvalidDevice;
     WaveIO1D io{y, ysrc, rX, rXsrc};
     hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
+#else
+    return hipErrorNotSupported;
+#endif
   } else {
     auto k = batched_kernel<M, LU_T, IO1D>;
     const int grid = resident_blocks(k, q.ncells);
@@ -76,11 +80,15 @@ hipError_t batched_box_mode(const ReactParamsDev& P, const BoxArgs& b, double dt
   using LU_T = typename std::conditional<MODE == MODE_BATCHED_MIXED, float, double>::type;
   using Tab = typename wave_tables<M>::type;
   if constexpr (!std::is_void<Tab>::value) {
+#if PPC_WAVE_BATCHED
     auto k = wave_bdf_kernel<Tab, LU_T, WaveIOBox>;
     const int grid = resident_blocks(k, q.ncells * 64);
     if (grid < 0) return hipErrorInvalidDevice;
     WaveIOBox io{b};
     hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
+#else
+    return hipErrorNotSupported;
+#endif
   } else {
     auto k = batched_kernel<M, LU_T, IOBox>;
     const int grid = resident_blocks(k, q.ncells);

@@ -12,30 +12,34 @@ namespace wave {
 
 PPC_DI int lane_id() { return (int)__lane_id(); }
 
+// (hi, lo) 32-bit halves of a double, and back, in unsigned arithmetic
+PPC_DI unsigned int lo32(double v) { return (unsigned int)(unsigned long long)__double_as_longlong(v); }
+PPC_DI unsigned int hi32(double v) { return (unsigned int)((unsigned long long)__double_as_longlong(v) >> 32); }
+PPC_DI double from32(unsigned int hi, unsigned int lo) {
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | (unsigned long long)lo));
+}
+
 PPC_DI int rdlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 PPC_DI float rdlane(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
 PPC_DI double rdlane(double v, int l) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)b, l);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  const unsigned int lo = (unsigned int)__builtin_amdgcn_readlane((int)lo32(v), l);
+  const unsigned int hi = (unsigned int)__builtin_amdgcn_readlane((int)hi32(v), l);
+  return from32(hi, lo);
 }
 // value of the first lane (all lanes must hold the same value already)
 PPC_DI double uni(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readfirstlane((int)b);
-  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  const unsigned int lo = (unsigned int)__builtin_amdgcn_readfirstlane((int)lo32(v));
+  const unsigned int hi = (unsigned int)__builtin_amdgcn_readfirstlane((int)hi32(v));
+  return from32(hi, lo);
 }
 PPC_DI int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // DPP lane exchange inside rows of 16 lanes
 template <int CTRL>
 PPC_DI double dpp(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  const unsigned int lo = (unsigned int)__builtin_amdgcn_update_dpp(0, (int)lo32(v), CTRL, 0xF, 0xF, false);
+  const unsigned int hi = (unsigned int)__builtin_amdgcn_update_dpp(0, (int)hi32(v), CTRL, 0xF, 0xF, false);
+  return from32(hi, lo);
 }
 template <int CTRL>
 PPC_DI float dpp(float v) {

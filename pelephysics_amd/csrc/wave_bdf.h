@@ -172,9 +172,12 @@ struct WaveChem {
 
   // fKernelSpec (Source/Reactions/ReactorUtils.H:358-415) for the cell of this
   // wave: y = (rhoY_lane | T in lane NS), returns ydot of this lane.
+  // Optional outputs (adjoint): the energy-inverted T, this lane's species
+  // energy e_k or h_k (erg/g) and rho*cv or rho*cp.
   template <class SH>
   PPC_DI static double rhs(const Lane& L, SH& sh, int lane, double dt_save, int reactor_type, double y,
-                           double rYsrc, double rhoe_init, double rhoesrc_ext) {
+                           double rYsrc, double rhoe_init, double rhoesrc_ext, double* Tout = nullptr,
+                           double* ekout = nullptr, double* rcout = nullptr) {
     const double rY = L.sp ? y : 0.0;
     const double rho = wave::sum(rY);
     const double rinv = 1.0 / rho;
@@ -202,6 +205,9 @@ struct WaveChem {
     wd = wd * 1.0e-6 * L.mw;
     const double v = L.sp ? wd + rYsrc : 0.0;
     const double src = rhoesrc_ext - wave::sum(v * ek);
+    if (Tout) *Tout = Tv;
+    if (ekout) *ekout = ek;
+    if (rcout) *rcout = rho * cmix;
     return L.sp ? v : (lane == NS ? src * (rinv / cmix) : 0.0);
   }
 

@@ -29,7 +29,8 @@ from typing import Optional, Sequence
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpele_chem_hip.so")
+# the in-tree build; PELE_CHEM_HIP_LIB selects another build of the same library
+LIB_PATH = os.environ.get("PELE_CHEM_HIP_LIB") or os.path.join(HERE, "libpele_chem_hip.so")
 
 BDF, RK64, BDF_ADAPTIVE, BDF_ADAPTIVE_MIXED, BDF_BATCHED_FP64 = 0, 1, 2, 3, 4
 E_REACTOR, H_REACTOR = 1, 2  # Source/Reactions/ReactorTypes.H:10-11
