@@ -150,6 +150,51 @@ def cpu_baseline(m, st):
                       f"rtol={RTOL:g} atol={ATOL:g}, {el:.1f} s"}
 
 
+def adjoint_measure(m):
+    """Adjoint workload: drm19, 4096 fresh CH4/air cells (T ~ U[1100,1500] K,
+    phi ~ U[0.6,1.4], 1 atm), 10 ReactorBDF calls of 1e-5 s, d(T, Y_CO, Y_OH)/d ln A_j
+    for all reactions.  Time of the adjoint kernel (forward with checkpoints +
+    reverse sweep) against the forward pass alone (the same 10 ReactorBDF calls)."""
+    import torch
+    from pelephysics_amd.reactor import ReactorBase
+    from pelephysics_amd import workloads as W
+    dev = torch.device("cuda:0")
+    st = W.drm19_fresh(m, 4096)
+    n = m.n_species + 1
+    y0 = torch.from_numpy(st.y_1d().reshape(st.ncells, n)).to(dev)
+    rX0 = torch.from_numpy(st.rE.copy()).to(dev)
+    R = ReactorBase.create("ReactorBDF")
+    R.init(2, st.ncells, mechanism=MECH)
+    R.adjoint_1d(y0.clone(), rX0, 10, DT)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out, grad = R.adjoint_1d(y0.clone(), rX0, 10, DT)
+    torch.cuda.synchronize()
+    adj_s = time.perf_counter() - t0
+    adj_kms = R.stats()["kernel_ms"]
+    ys = torch.zeros(st.ncells * m.n_species, dtype=torch.float64, device=dev)
+    zs = torch.zeros(st.ncells, dtype=torch.float64, device=dev)
+    y, rX = y0.clone().reshape(-1), rX0.clone()
+    R.react_1d(y, ys, rX, zs, DT)
+    torch.cuda.synchronize()
+    y, rX = y0.clone().reshape(-1), rX0.clone()
+    t0 = time.perf_counter()
+    fwd_kms = 0.0
+    for _ in range(10):
+        R.react_1d(y, ys, rX, zs, DT)
+        fwd_kms += R.stats()["kernel_ms"]
+    torch.cuda.synchronize()
+    fwd_s = time.perf_counter() - t0
+    R.close()
+    return {"workload": "drm19 4096 fresh cells, 10 ReactorBDF calls of 1e-5 s, d(T,Y_CO,Y_OH)/d lnA_j, 84 reactions",
+            "adjoint_kernel_ms": adj_kms, "adjoint_wall_ms": adj_s * 1e3,
+            "forward_kernel_ms": fwd_kms, "forward_wall_ms": fwd_s * 1e3,
+            "time_ratio": adj_kms / fwd_kms if fwd_kms > 0 else None,
+            "checkpoint_bytes": st.ncells * 10 * n * 8, "state_bytes": st.ncells * n * 8,
+            "grad_bytes": st.ncells * m.n_reactions * 3 * 8,
+            "finite": bool(torch.isfinite(grad).all().item())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +203,7 @@ def main():
     ap.add_argument("--mode", choices=["mixed", "batched-fp64", "fp64"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only copy this many bytes on the device (counter calibration)")
     a = ap.parse_args()
@@ -195,6 +241,9 @@ def main():
     rl = roofline(m, s, head["kernel_ms"], mixed)
     rl["traffic"] = measured_traffic(ident)
     rl["algorithmic_bytes"] = (3 * m.n_species + 6) * 8 * ncells  # (3N+6)*8 B per cell: read+write once
+    adj = None if a.no_adjoint else adjoint_measure(m)
+    if adj:
+        log(f"[bench] adjoint: {adj['adjoint_kernel_ms']:.1f} ms vs forward {adj['forward_kernel_ms']:.1f} ms")
     cpu = None if a.no_cpu else cpu_baseline(m, st)
     if cpu:
         log(f"[bench] cpu baseline {cpu['value']:.4g} cells/s on {cpu['cores']} threads")
@@ -212,6 +261,7 @@ def main():
         "kernel_ms": head["kernel_ms"],
         "roofline": rl, "cpu_baseline": cpu,
     }
+    line["adjoint"] = adj
     line.update(extra)
     print(json.dumps(line))
 

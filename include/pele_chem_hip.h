@@ -108,6 +108,21 @@ int pp_chem_react_1d(pp_chem* h, double* y, double* ysrc, double* rX, double* rX
                      double* dt_react, double* time, int ncells, int on_device, void* stream);
 
 int pp_chem_get_stats(const pp_chem* h, pp_chem_stats* out);
+int pp_chem_num_reactions(const pp_chem* h);
+
+/* Discrete adjoint of the fixed-step BDF integration: no counterpart in the
+ * reference (it exposes no sensitivities); the map is that of
+ * ReactorBDF::react (ReactorBDF.cpp:187-285) with the handle's ode.bdf_*
+ * parameters (bdf_nonlinear_iters must be 1), called nmacro times with dt each
+ * (1-D form, no sources), with every linear system solved exactly.
+ * y: ncells x (N+1) YCOrder state (device), overwritten with the final state;
+ * rX: ncells rho e or rho h (device); out: ncells x 3 (device) = T_final,
+ * Y[spec_a], Y[spec_b]; grad: ncells x NR x 3 (device) = d out / d ln A_j,
+ * A_j the pre-exponential of reaction j in mechanism order (for falloff
+ * reactions the high- and low-pressure limits together).  The device time of
+ * the kernel is in pp_chem_get_stats().kernel_ms. */
+int pp_chem_adjoint_1d(pp_chem* h, double* y, const double* rX, int ncells, int nmacro, double dt,
+                       int spec_a, int spec_b, double* out, double* grad, void* stream);
 /* Per-cell accepted step counts of the last call (host copy). */
 int pp_chem_copy_cell_steps(pp_chem* h, int* host_out, int ncells);
 const char* pp_chem_last_error(const pp_chem* h);

@@ -153,6 +153,9 @@ struct pp_chem {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   pp_chem_stats stats{};
   std::string err;
+  // adjoint: substep checkpoints (device), grown on demand
+  size_t ckpt_cap = 0;
+  double* d_ckpt = nullptr;
 };
 
 namespace {
@@ -364,6 +367,7 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
 }
 
 void pp_chem_destroy(pp_chem* h) {
+  if (h && h->d_ckpt) (void)hipFree(h->d_ckpt);
   if (!h) return;
   if (h->d_cnt) (void)hipFree(h->d_cnt);
   if (h->d_red) (void)hipFree(h->d_red);
@@ -376,6 +380,45 @@ void pp_chem_destroy(pp_chem* h) {
 }
 
 int pp_chem_num_species(const pp_chem* h) { return h ? h->L->ns : PP_CHEM_EINVAL; }
+int pp_chem_num_reactions(const pp_chem* h) { return h ? h->L->nr : PP_CHEM_EINVAL; }
+
+int pp_chem_adjoint_1d(pp_chem* h, double* y, const double* rX, int ncells, int nmacro, double dt,
+                       int spec_a, int spec_b, double* out, double* grad, void* stream) {
+  if (!h) return PP_CHEM_EINVAL;
+  if (ncells < 0 || nmacro < 1 || !(dt > 0)) return fail(h, PP_CHEM_EINVAL, "bad argument");
+  if (ncells > 0 && (!y || !rX || !out || !grad)) return fail(h, PP_CHEM_EINVAL, "null array");
+  if (spec_a < 0 || spec_a >= h->L->ns || spec_b < 0 || spec_b >= h->L->ns)
+    return fail(h, PP_CHEM_EINVAL, "output species index out of range");
+  if (h->cfg.bdf_nonlinear_iters != 1)
+    return fail(h, PP_CHEM_EINVAL, "the adjoint differentiates one Newton iteration per substep");
+  // the adjoint runs on the table-driven form of the mechanism
+  const MechLauncher* L = h->L;
+  if (L->nr > 128) return fail(h, PP_CHEM_EINVAL, "more than 128 reactions");
+  hipStream_t s = (hipStream_t)stream;
+  if (ncells == 0) return 0;
+  const std::string tab = std::string(L->name) + "_table";
+  const MechLauncher* LA = find_mech(tab.c_str());
+  if (!LA) LA = L;
+  const size_t need = (size_t)ncells * (size_t)nmacro * (size_t)h->cfg.bdf_nsubsteps * (size_t)(L->ns + 1);
+  if (need > h->ckpt_cap) {
+    if (h->d_ckpt) HIPCHK(h, hipFree(h->d_ckpt));
+    h->d_ckpt = nullptr;
+    HIPCHK(h, hipMalloc(&h->d_ckpt, need * sizeof(double)));
+    h->ckpt_cap = need;
+  }
+  ppc::AdjointArgs a{y, rX, grad, out, h->d_ckpt, ncells, nmacro, spec_a, spec_b, dt};
+  HIPCHK(h, hipEventRecord(h->ev0, s));
+  const hipError_t e = LA->adjoint_1d(h->P, a, s);
+  if (e == hipErrorNotSupported) return fail(h, PP_CHEM_EINVAL, "no table-driven form of " + std::string(L->name));
+  if (e != hipSuccess) return hip_fail(h, e, "adjoint launch");
+  HIPCHK(h, hipEventRecord(h->ev1, s));
+  HIPCHK(h, hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->stats.kernel_ms = ms;
+  h->stats.ncells = ncells;
+  return 0;
+}
 
 int pp_chem_react_box(pp_chem* h, const int lo[3], const int hi[3], const pp_array4* rY,
                       const pp_array4* rYsrc, const pp_array4* T, const pp_array4* rE,

@@ -21,7 +21,7 @@ struct is_table_mech<M, std::void_t<decltype(M::kTable)>> : std::true_type {};
 
 struct MechLauncher {
   const char* name;
-  int ns;
+  int ns, nr;
   int batched_ok;  // the batched modes are available
   hipError_t (*react_1d)(int mode, const ReactParamsDev& P, double* y, const double* ysrc, double* rX,
                          const double* rXsrc, double dt, int ncells, CellStats st, int block,
@@ -33,7 +33,12 @@ struct MechLauncher {
                            const double* rXsrc, double dt, QueueArgs q, CellStats st, hipStream_t s);
   hipError_t (*batched_box)(int mode, const ReactParamsDev& P, const BoxArgs& b, double dt, QueueArgs q,
                             CellStats st, hipStream_t s);
+  // discrete adjoint of the fixed-step BDF (table-driven mechanisms only)
+  hipError_t (*adjoint_1d)(const ReactParamsDev& P, const AdjointArgs& a, hipStream_t s);
 };
+
+template <class M>
+hipError_t launch_adjoint(const ReactParamsDev& P, const AdjointArgs& a, hipStream_t s);
 
 template <class M, int MODE>
 hipError_t launch_1d_mode(const ReactParamsDev& P, double* y, const double* ysrc, double* rX,
@@ -93,10 +98,11 @@ hipError_t launch_box(int mode, const ReactParamsDev& P, const BoxArgs& b, doubl
 #define PPC_DEFINE_LAUNCHER(NAME, TYPE)                                                        \
   namespace ppc {                                                                              \
   const MechLauncher* launcher_##NAME() {                                                      \
-    static const MechLauncher L = {#NAME, TYPE::NS,                                            \
+    static const MechLauncher L = {#NAME, TYPE::NS, TYPE::NR,                                  \
                                    !is_table_mech<TYPE>::value || PPC_WAVE_BATCHED,            \
                                    &launch_1d<TYPE>, &launch_box<TYPE>,                        \
-                                   &launch_batched_1d<TYPE>, &launch_batched_box<TYPE>};      \
+                                   &launch_batched_1d<TYPE>, &launch_batched_box<TYPE>,        \
+                                   &launch_adjoint<TYPE>};                                     \
     return &L;                                                                                 \
   }                                                                                            \
   }

@@ -5,6 +5,7 @@
 #include "launch.h"
 #include "batched.h"
 #include "wave_bdf.h"
+#include "wave_adjoint.h"
 
 namespace ppc {
 
@@ -99,7 +100,26 @@ hipError_t batched_box_mode(const ReactParamsDev& P, const BoxArgs& b, double dt
   return hipGetLastError();
 }
 
+template <class M>
+hipError_t launch_adjoint(const ReactParamsDev& P, const AdjointArgs& a, hipStream_t s) {
+  using Tab = typename wave_tables<M>::type;
+  if constexpr (!std::is_void<Tab>::value) {
+    auto k = wave_adjoint_kernel<Tab>;
+    const int grid = resident_blocks(k, (long long)a.ncells * 64);
+    if (grid < 0) return hipErrorInvalidDevice;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, a);
+    return hipGetLastError();
+  } else {
+    return hipErrorNotSupported;
+  }
+}
+
 }  // namespace ppc
+
+#define PPC_INSTANTIATE_ADJOINT(TYPE, MODE)                                                           \
+  namespace ppc {                                                                                    \
+  template hipError_t launch_adjoint<TYPE>(const ReactParamsDev&, const AdjointArgs&, hipStream_t);  \
+  }
 
 #define PPC_INSTANTIATE_MODE(TYPE, MODE)                                                              \
   namespace ppc {                                                                                    \

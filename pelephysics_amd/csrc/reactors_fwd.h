@@ -31,6 +31,17 @@ struct QueueArgs {
   int ncells;
 };
 
+// Arguments of the discrete-adjoint kernel (wave_adjoint.h); device pointers.
+struct AdjointArgs {
+  double* y;          // ncells x NEQ (YCOrder): initial state in, final state out
+  const double* rX;   // ncells: rho e (reactor_type 1) or rho h (2); no sources
+  double* grad;       // ncells x NR x 3: d(T_final, Y_a, Y_b)/d ln A_j, reactions in mechanism order
+  double* out;        // ncells x 3: T_final, Y_a, Y_b
+  double* ckpt;       // ncells x (nmacro * nsubsteps) x NEQ checkpoints
+  int ncells, nmacro, spec_a, spec_b;
+  double dt;          // per call
+};
+
 struct CellStats {
   int* nsteps;  // per cell (always)
   int* nrhs;    // optional per-cell counters (may be null)

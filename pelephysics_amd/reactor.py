@@ -111,6 +111,9 @@ def library() -> C.CDLL:
         L.pp_chem_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
         L.pp_chem_destroy.argtypes = [vp]
         L.pp_chem_num_species.argtypes = [vp]
+        L.pp_chem_num_reactions.argtypes = [vp]
+        L.pp_chem_adjoint_1d.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, vp, vp,
+                                         vp]
         L.pp_chem_list_mechanisms.argtypes = [C.c_char_p, C.c_int]
         a4 = C.POINTER(Array4)
         L.pp_chem_react_box.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), a4, a4, a4, a4,
@@ -248,6 +251,28 @@ class ReactorBase:
         rc = library().pp_chem_react_1d(self._h, *ptrs, C.byref(dt), C.byref(tm), ncells, on_dev,
                                         stream)
         return self._check(rc), tm.value
+
+    def adjoint_1d(self, y, rX, nmacro: int, dt_react: float, species=("CO", "OH"),
+                   stream: Optional[int] = None):
+        """Fixed-step BDF over nmacro calls of dt_react (this handle's ode.bdf_*
+        keys, exact linear solves) with its discrete adjoint.  y: (ncells, N+1)
+        float64 device tensor, overwritten with the final state; rX: (ncells,)
+        rho e / rho h.  Returns (out, grad): out (ncells, 3) = T_final, Y_a, Y_b;
+        grad (ncells, NR, 3) = d out / d ln A_j, j in mechanism order."""
+        import torch
+        from .mechgen.__main__ import load
+        names = [sp.name for sp in load(self.mechanism.replace("_table", "")).species]
+        sa, sb = (names.index(k) for k in species)
+        nr = library().pp_chem_num_reactions(self._h)
+        ncells = int(rX.shape[0])
+        if not (y.is_contiguous() and rX.is_contiguous()):
+            raise ValueError("device tensors must be contiguous")
+        out = torch.empty((ncells, 3), dtype=torch.float64, device=y.device)
+        grad = torch.empty((ncells, nr, 3), dtype=torch.float64, device=y.device)
+        rc = library().pp_chem_adjoint_1d(self._h, y.data_ptr(), rX.data_ptr(), ncells, nmacro, dt_react, sa, sb,
+                                          out.data_ptr(), grad.data_ptr(), stream)
+        self._check(rc)
+        return out, grad
 
     def stats(self) -> dict:
         s = Stats()
