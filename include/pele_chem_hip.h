@@ -65,6 +65,11 @@ typedef struct {
   double rtol, atol, first_step;
   int max_steps;
   int block_size; /* threads per block, 0: default (64) */
+  /* Upper bound on the device workspace that holds the per-cell matrices of
+   * mechanisms with more than 31 species (Jacobian, LU, Krylov basis; too large
+   * for a lane's private memory); 0: default (32 GiB).  Fewer resident cells
+   * are integrated at once when it is reached. */
+  long long workspace_bytes;
 } pp_chem_config;
 
 /* amrex::Array4<Real> layout: element (i,j,k,n) is at
@@ -123,6 +128,10 @@ int pp_chem_num_reactions(const pp_chem* h);
  * the kernel is in pp_chem_get_stats().kernel_ms. */
 int pp_chem_adjoint_1d(pp_chem* h, double* y, const double* rX, int ncells, int nmacro, double dt,
                        int spec_a, int spec_b, double* out, double* grad, void* stream);
+/* Doubles per resident cell of the per-cell matrix workspace the handle's
+ * integrator uses (0: the matrices stay in private memory).  No counterpart in
+ * the reference; for sizing pp_chem_config.workspace_bytes. */
+long long pp_chem_workspace_rows(const pp_chem* h);
 /* Per-cell accepted step counts of the last call (host copy). */
 int pp_chem_copy_cell_steps(pp_chem* h, int* host_out, int ncells);
 const char* pp_chem_last_error(const pp_chem* h);

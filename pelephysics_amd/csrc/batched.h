@@ -95,11 +95,17 @@ __global__ void __launch_bounds__(64) PPC_LANE_OCC batched_kernel(ReactParamsDev
   const double newton_tol = fmax(10 * 2.220446049250313e-16 / rtol, fmin(0.03, sqrt(rtol)));
   const double t_bound = dt;
 
-  // per-lane integrator state
+  // per-lane integrator state; the matrices of the larger mechanisms live in
+  // the workspace (lane_ws.h), rows [LU | Jf | Jd]
+  constexpr bool WS = uses_ws<M>();
+  const LaneWS w{P.ws, P.ws_lanes, (long long)blockIdx.x * blockDim.x + threadIdx.x};
   double D[(AMAXORD + 3) * n];
-  LU_T LU[n * n];
-  float Jf[n * n];  // Jacobian kept for LU refreshes (FP32 storage)
-  double Jd[n * n];  // FP64 workspace of the Jacobian evaluation
+  LU_T LU_p[WS ? 1 : n * n];
+  float Jf_p[WS ? 1 : n * n];
+  double Jd_p[WS ? 1 : n * n];
+  const auto LU = lane_array<WS>(LU_p, w, 0);
+  const auto Jf = lane_array<WS>(Jf_p, w, ws_rows<LU_T>(n * n));  // Jacobian kept for LU refreshes (FP32)
+  const auto Jd = lane_array<WS>(Jd_p, w, ws_rows<LU_T>(n * n) + ws_rows<float>(n * n));  // FP64 evaluation
   int piv[n];
   double yv[n], f[n], d[n], ypred[n], psi[n], scl[n], lus[n], src[NS];
   double rhoe = 0, rhoesrc = 0;
@@ -147,9 +153,18 @@ __global__ void __launch_bounds__(64) PPC_LANE_OCC batched_kernel(ReactParamsDev
     for (int i = 0; i < n; ++i) lus[i] = scl[i];
     for (int i = 0; i < n; ++i) {
       const double si = 1.0 / lus[i];
-      for (int j = 0; j < n; ++j) {
-        const double m = (i == j ? 1.0 : 0.0) - c * (double)Jf[i * n + j];
-        LU[i * n + j] = (LU_T)(si * m * lus[j]);
+      for (int j0 = 0; j0 < n; j0 += LU_CHUNK) {  // loads of a chunk before its stores
+        float jf[LU_CHUNK];
+#pragma unroll
+        for (int u = 0; u < LU_CHUNK; ++u)
+          if (j0 + u < n) jf[u] = Jf[i * n + j0 + u];
+#pragma unroll
+        for (int u = 0; u < LU_CHUNK; ++u)
+          if (j0 + u < n) {
+            const int j = j0 + u;
+            const double m = (i == j ? 1.0 : 0.0) - c * (double)jf[u];
+            LU[i * n + j] = (LU_T)(si * m * lus[j]);
+          }
       }
     }
     lu_factor<n, LU_T>(LU, piv);
@@ -159,7 +174,15 @@ __global__ void __launch_bounds__(64) PPC_LANE_OCC batched_kernel(ReactParamsDev
 
   auto jacobian = [&](const double* yj) {
     bdf_jacobian<M>(P.reactor_type, yj, 1.0, Jd, Jd);
-    for (int i = 0; i < n * n; ++i) Jf[i] = (float)Jd[i];
+    for (int i0 = 0; i0 < n * n; i0 += LU_CHUNK) {
+      double v[LU_CHUNK];
+#pragma unroll
+      for (int u = 0; u < LU_CHUNK; ++u)
+        if (i0 + u < n * n) v[u] = Jd[i0 + u];
+#pragma unroll
+      for (int u = 0; u < LU_CHUNK; ++u)
+        if (i0 + u < n * n) Jf[i0 + u] = (float)v[u];
+    }
     njac++;
   };
 

@@ -156,6 +156,9 @@ struct pp_chem {
   // adjoint: substep checkpoints (device), grown on demand
   size_t ckpt_cap = 0;
   double* d_ckpt = nullptr;
+  // per-lane matrix workspace of the larger mechanisms, grown on demand
+  size_t ws_cap = 0;
+  double* d_ws = nullptr;
 };
 
 namespace {
@@ -231,6 +234,37 @@ int finish(pp_chem* h, int ncells, hipStream_t s, float kernel_ms) {
 }
 
 bool is_batched(int mode) { return mode == ppc::MODE_BATCHED_MIXED || mode == ppc::MODE_BATCHED_FP64; }
+
+// Per-lane workspace of mechanisms too large for private memory: one row of
+// doubles per lane and matrix element, for at most 8 waves per CU (the
+// resident waves of the 2-waves-per-SIMD kernels) and at most
+// cfg.workspace_bytes (default kWsMaxBytes).
+constexpr long long kWsMaxBytes = 32LL << 30;
+
+int prepare_workspace(pp_chem* h, int ncells) {
+  h->P.ws = nullptr;
+  h->P.ws_lanes = 0;
+  const long long rows = h->L->ws_rows(h->cfg.mode);
+  if (rows == 0 || ncells == 0) return 0;
+  int ncu = 0;
+  HIPCHK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+  long long blocks = std::min<long long>((ncells + 63) / 64, 8LL * std::max(ncu, 1));
+  const long long cap = h->cfg.workspace_bytes > 0 ? h->cfg.workspace_bytes : kWsMaxBytes;
+  const long long lane_bytes = 64 * rows * (long long)sizeof(double);  // one 64-lane block
+  if (lane_bytes > cap) return fail(h, PP_CHEM_EINVAL, "workspace_bytes below one 64-cell block");
+  blocks = std::min(blocks, cap / lane_bytes);
+  const size_t need = (size_t)(blocks * 64 * rows);
+  if (need > h->ws_cap) {
+    if (h->d_ws) HIPCHK(h, hipFree(h->d_ws));
+    h->d_ws = nullptr;
+    h->ws_cap = 0;
+    HIPCHK(h, hipMalloc(&h->d_ws, need * sizeof(double)));
+    h->ws_cap = need;
+  }
+  h->P.ws = h->d_ws;
+  h->P.ws_lanes = blocks * 64;
+  return 0;
+}
 
 // Prepares the work queue: zero the counter and, when the previous call
 // integrated the same cells, order them by that call's RHS counts.
@@ -344,6 +378,7 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
   if (cfg->mode >= PP_CHEM_BDF_ADAPTIVE &&
       (!(cfg->rtol > 0) || !(cfg->atol > 0) || cfg->max_steps < 1))
     return PP_CHEM_EINVAL;
+  if (cfg->workspace_bytes < 0) return PP_CHEM_EINVAL;
   const int bs = cfg->block_size > 0 ? cfg->block_size : 64;
   if (bs > 64 || bs % 64 != 0) return PP_CHEM_EINVAL;  // kernels are built for 64-lane blocks
   pp_chem* h = new pp_chem();
@@ -367,8 +402,9 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
 }
 
 void pp_chem_destroy(pp_chem* h) {
-  if (h && h->d_ckpt) (void)hipFree(h->d_ckpt);
   if (!h) return;
+  if (h->d_ckpt) (void)hipFree(h->d_ckpt);
+  if (h->d_ws) (void)hipFree(h->d_ws);
   if (h->d_cnt) (void)hipFree(h->d_cnt);
   if (h->d_red) (void)hipFree(h->d_red);
   if (h->d_queue) (void)hipFree(h->d_queue);
@@ -381,6 +417,7 @@ void pp_chem_destroy(pp_chem* h) {
 
 int pp_chem_num_species(const pp_chem* h) { return h ? h->L->ns : PP_CHEM_EINVAL; }
 int pp_chem_num_reactions(const pp_chem* h) { return h ? h->L->nr : PP_CHEM_EINVAL; }
+long long pp_chem_workspace_rows(const pp_chem* h) { return h ? h->L->ws_rows(h->cfg.mode) : PP_CHEM_EINVAL; }
 
 int pp_chem_adjoint_1d(pp_chem* h, double* y, const double* rX, int ncells, int nmacro, double dt,
                        int spec_a, int spec_b, double* out, double* grad, void* stream) {
@@ -394,6 +431,7 @@ int pp_chem_adjoint_1d(pp_chem* h, double* y, const double* rX, int ncells, int 
   // the adjoint runs on the table-driven form of the mechanism
   const MechLauncher* L = h->L;
   if (L->nr > 128) return fail(h, PP_CHEM_EINVAL, "more than 128 reactions");
+  if (L->ns + 1 > 64) return fail(h, PP_CHEM_EINVAL, "more than 63 species");
   hipStream_t s = (hipStream_t)stream;
   if (ncells == 0) return 0;
   const std::string tab = std::string(L->name) + "_table";
@@ -456,6 +494,7 @@ int pp_chem_react_box(pp_chem* h, const int lo[3], const int hi[3], const pp_arr
   b.rEsrc = to_a4(rEsrc);
   b.FC = to_a4(FC);
   if (int r = ensure_counters(h, ncells > 0 ? ncells : 1)) return r;
+  if (int r = prepare_workspace(h, ncells)) return r;
   hipStream_t s = (hipStream_t)stream;
   HIPCHK(h, hipEventRecord(h->ev0, s));
   if (ncells > 0) {
@@ -485,6 +524,7 @@ int pp_chem_react_1d(pp_chem* h, double* y, double* ysrc, double* rX, double* rX
   const int ns = h->L->ns, n = ns + 1;
   hipStream_t s = (hipStream_t)stream;
   if (int r = ensure_counters(h, ncells > 0 ? ncells : 1)) return r;
+  if (int r = prepare_workspace(h, ncells)) return r;
   double *dy = y, *dys = ysrc, *dx = rX, *dxs = rXsrc;
   const size_t ny = (size_t)ncells * n, nys = (size_t)ncells * ns, nx = (size_t)ncells;
   if (!on_device && ncells > 0) {

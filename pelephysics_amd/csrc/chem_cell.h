@@ -125,10 +125,11 @@ PPC_HEAVY void f_kernel_spec(const double dt_save, const int reactor_type, const
 // The BDF Jacobian in (rhoY, T) variables, row-major A[i*n+j] = s * d f_i / d y_j
 // (ReactorBDF.cpp:67-82: RTY2JAC -> DWDOT at the integrated T, species block
 // scaled by mw_i/mw_j, T column by mw_i, T row by 1/mw_j).  Jc is a
-// scratch array of n*n doubles; A may alias Jc.
-template <class M>
+// scratch array of n*n doubles; A may alias Jc.  JM is double* or a
+// LaneArr<double> (lane_ws.h).
+template <class M, class JM>
 PPC_HEAVY void bdf_jacobian(const int reactor_type, const double* __restrict__ y, const double s,
-                         double* Jc, double* A) {
+                         const JM Jc, const JM A) {
   constexpr int NS = M::NS, n = NS + 1;
   double sc[NS];
 #pragma unroll
@@ -143,18 +144,32 @@ PPC_HEAVY void bdf_jacobian(const int reactor_type, const double* __restrict__ y
     Jc[NS * n + k] *= 1.0e-6;
     Jc[k * n + NS] *= 1.0e6;
   }
-  // transpose (column-major -> row-major) with molecular-weight scaling, in place
+  // transpose (column-major -> row-major) with molecular-weight scaling, in
+  // place; chunks of 8 pairs load before they store (Jc and A may alias)
+  constexpr int U = 8;
   for (int i = 0; i < n; ++i) {
     const double wi = (i < NS) ? M::mw_rt(i) : 1.0;
     const double iwi = (i < NS) ? M::imw_rt(i) : 1.0;
     A[i * n + i] = s * Jc[i * n + i];
-    for (int j = i + 1; j < n; ++j) {
-      const double wj = (j < NS) ? M::mw_rt(j) : 1.0;
-      const double iwj = (j < NS) ? M::imw_rt(j) : 1.0;
-      const double a_ij = Jc[j * n + i] * wi * iwj;  // d f_i / d y_j
-      const double a_ji = Jc[i * n + j] * wj * iwi;  // d f_j / d y_i
-      A[i * n + j] = s * a_ij;
-      A[j * n + i] = s * a_ji;
+    for (int j0 = i + 1; j0 < n; j0 += U) {
+      double cji[U], cij[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (j0 + u < n) {
+          cji[u] = Jc[(j0 + u) * n + i];
+          cij[u] = Jc[i * n + j0 + u];
+        }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (j0 + u < n) {
+          const int j = j0 + u;
+          const double wj = (j < NS) ? M::mw_rt(j) : 1.0;
+          const double iwj = (j < NS) ? M::imw_rt(j) : 1.0;
+          const double a_ij = cji[u] * wi * iwj;  // d f_i / d y_j
+          const double a_ji = cij[u] * wj * iwi;  // d f_j / d y_i
+          A[i * n + j] = s * a_ij;
+          A[j * n + i] = s * a_ji;
+        }
     }
   }
 }

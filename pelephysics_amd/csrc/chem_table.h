@@ -402,8 +402,9 @@ struct TabMech {
   // Analytic Jacobian in SI-mol units, column-major J[k*(NS+1)+m] = d wdot_m / d c_k,
   // J[NS*(NS+1)+m] = d wdot_m / dT, T row d Tdot / d c_k, d Tdot / dT
   // (aJacobian semantics, Mechanisms/LiDryer/mechanism.H:3905-5138).
-  template <int CONSP>
-  PPC_DI static void jacobian(double* __restrict__ J, const double* __restrict__ sc, const double Tv) {
+  // J is a double* or a LaneArr<double> (lane_ws.h).
+  template <int CONSP, class JM>
+  PPC_DI static void jacobian(const JM J, const double* __restrict__ sc, const double Tv) {
     constexpr int n = NS + 1;
 #pragma unroll 1
     for (int i = 0; i < n * n; ++i) J[i] = 0.0;

@@ -5,11 +5,13 @@
 #include <hip/hip_runtime.h>
 #include <type_traits>
 #include "reactors_fwd.h"
+#include "lane_ws.h"
 
 namespace ppc {
 
-// The batched forms of table-driven mechanisms run one wave per cell
-// (wave_bdf.h); that form is compiled only with PPC_WAVE_BATCHED=1.
+// The one-wave-per-cell batched form of table-driven mechanisms (wave_bdf.h)
+// is compiled only with PPC_WAVE_BATCHED=1; otherwise every mechanism runs
+// the batched forms one lane per cell (batched.h).
 #ifndef PPC_WAVE_BATCHED
 #define PPC_WAVE_BATCHED 0
 #endif
@@ -35,6 +37,8 @@ struct MechLauncher {
                             CellStats st, hipStream_t s);
   // discrete adjoint of the fixed-step BDF (table-driven mechanisms only)
   hipError_t (*adjoint_1d)(const ReactParamsDev& P, const AdjointArgs& a, hipStream_t s);
+  // workspace rows per lane a mode needs (lane_ws.h); 0: none
+  long long (*ws_rows)(int mode);
 };
 
 template <class M>
@@ -99,10 +103,10 @@ hipError_t launch_box(int mode, const ReactParamsDev& P, const BoxArgs& b, doubl
   namespace ppc {                                                                              \
   const MechLauncher* launcher_##NAME() {                                                      \
     static const MechLauncher L = {#NAME, TYPE::NS, TYPE::NR,                                  \
-                                   !is_table_mech<TYPE>::value || PPC_WAVE_BATCHED,            \
+                                   1,                                                          \
                                    &launch_1d<TYPE>, &launch_box<TYPE>,                        \
                                    &launch_batched_1d<TYPE>, &launch_batched_box<TYPE>,        \
-                                   &launch_adjoint<TYPE>};                                     \
+                                   &launch_adjoint<TYPE>, &integrator_ws_rows<TYPE>};          \
     return &L;                                                                                 \
   }                                                                                            \
   }

@@ -9,8 +9,8 @@
 
 namespace ppc {
 
-// Table-driven mechanisms run the batched forms one wave per cell
-// (wave_bdf.h); generated unrolled mechanisms one lane per cell (batched.h).
+// The batched forms run one lane per cell (batched.h); with PPC_WAVE_BATCHED=1
+// table-driven mechanisms run them one wave per cell (wave_bdf.h) instead.
 template <class M>
 struct wave_tables {
   using type = void;
@@ -20,19 +20,33 @@ struct wave_tables<TabMech<T>> {
   using type = T;
 };
 
+// Blocks of the one-lane-per-cell kernels: one lane per cell, or, with a
+// workspace, exactly the lanes it holds (the lanes then stride over the cells).
+template <class M, int MODE>
+int lane_grid(const ReactParamsDev& P, int ncells, int block) {
+  if (integrator_ws_rows<M>(MODE) > 0) {
+    if (!P.ws || P.ws_lanes < block || P.ws_lanes % block != 0) return -1;
+    return (int)(P.ws_lanes / block);
+  }
+  return (ncells + block - 1) / block;
+}
+
 template <class M, int MODE>
 hipError_t launch_1d_mode(const ReactParamsDev& P, double* y, const double* ysrc, double* rX,
                           const double* rXsrc, double dt, int ncells, CellStats st, int block, hipStream_t s) {
-  const dim3 grid((ncells + block - 1) / block), blk(block);
-  hipLaunchKernelGGL((react_1d_kernel<M, MODE>), grid, blk, 0, s, P, y, ysrc, rX, rXsrc, dt, ncells, st);
+  const int grid = lane_grid<M, MODE>(P, ncells, block);
+  if (grid < 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((react_1d_kernel<M, MODE>), dim3(grid), dim3(block), 0, s, P, y, ysrc, rX, rXsrc, dt,
+                     ncells, st);
   return hipGetLastError();
 }
 
 template <class M, int MODE>
 hipError_t launch_box_mode(const ReactParamsDev& P, const BoxArgs& b, double dt, int ncells, CellStats st,
                            int block, hipStream_t s) {
-  const dim3 grid((ncells + block - 1) / block), blk(block);
-  hipLaunchKernelGGL((react_box_kernel<M, MODE>), grid, blk, 0, s, P, b, dt, ncells, st);
+  const int grid = lane_grid<M, MODE>(P, ncells, block);
+  if (grid < 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((react_box_kernel<M, MODE>), dim3(grid), dim3(block), 0, s, P, b, dt, ncells, st);
   return hipGetLastError();
 }
 
@@ -54,21 +68,24 @@ template <class M, int MODE>
 hipError_t batched_1d_mode(const ReactParamsDev& P, double* y, const double* ysrc, double* rX, const double* rXsrc,
                            double dt, QueueArgs q, CellStats st, hipStream_t s) {
   using LU_T = typename std::conditional<MODE == MODE_BATCHED_MIXED, float, double>::type;
+#if PPC_WAVE_BATCHED
   using Tab = typename wave_tables<M>::type;
   if constexpr (!std::is_void<Tab>::value) {
-#if PPC_WAVE_BATCHED
     auto k = wave_bdf_kernel<Tab, LU_T, WaveIO1D>;
     const int grid = resident_blocks(k, q.ncells * 64);
     if (grid < 0) return hipErrorInvalidDevice;
     WaveIO1D io{y, ysrc, rX, rXsrc};
     hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
-#else
-    return hipErrorNotSupported;
+  } else
 #endif
-  } else {
+  {
     auto k = batched_kernel<M, LU_T, IO1D>;
-    const int grid = resident_blocks(k, q.ncells);
+    int grid = resident_blocks(k, q.ncells);
     if (grid < 0) return hipErrorInvalidDevice;
+    if (integrator_ws_rows<M>(MODE) > 0) {  // at most the lanes the workspace holds
+      if (!P.ws || P.ws_lanes < 64 || P.ws_lanes % 64 != 0) return hipErrorInvalidValue;
+      grid = (int)(grid < P.ws_lanes / 64 ? grid : P.ws_lanes / 64);
+    }
     IO1D io{y, ysrc, rX, rXsrc};
     hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
   }
@@ -79,21 +96,24 @@ template <class M, int MODE>
 hipError_t batched_box_mode(const ReactParamsDev& P, const BoxArgs& b, double dt, QueueArgs q, CellStats st,
                             hipStream_t s) {
   using LU_T = typename std::conditional<MODE == MODE_BATCHED_MIXED, float, double>::type;
+#if PPC_WAVE_BATCHED
   using Tab = typename wave_tables<M>::type;
   if constexpr (!std::is_void<Tab>::value) {
-#if PPC_WAVE_BATCHED
     auto k = wave_bdf_kernel<Tab, LU_T, WaveIOBox>;
     const int grid = resident_blocks(k, q.ncells * 64);
     if (grid < 0) return hipErrorInvalidDevice;
     WaveIOBox io{b};
     hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
-#else
-    return hipErrorNotSupported;
+  } else
 #endif
-  } else {
+  {
     auto k = batched_kernel<M, LU_T, IOBox>;
-    const int grid = resident_blocks(k, q.ncells);
+    int grid = resident_blocks(k, q.ncells);
     if (grid < 0) return hipErrorInvalidDevice;
+    if (integrator_ws_rows<M>(MODE) > 0) {  // at most the lanes the workspace holds
+      if (!P.ws || P.ws_lanes < 64 || P.ws_lanes % 64 != 0) return hipErrorInvalidValue;
+      grid = (int)(grid < P.ws_lanes / 64 ? grid : P.ws_lanes / 64);
+    }
     IOBox io{b};
     hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
   }
@@ -103,7 +123,8 @@ hipError_t batched_box_mode(const ReactParamsDev& P, const BoxArgs& b, double dt
 template <class M>
 hipError_t launch_adjoint(const ReactParamsDev& P, const AdjointArgs& a, hipStream_t s) {
   using Tab = typename wave_tables<M>::type;
-  if constexpr (!std::is_void<Tab>::value) {
+  // one equation per lane, at most two reactions per lane
+  if constexpr (!std::is_void<Tab>::value && M::NEQ <= 64 && M::NR <= 128) {
     auto k = wave_adjoint_kernel<Tab>;
     const int grid = resident_blocks(k, (long long)a.ncells * 64);
     if (grid < 0) return hipErrorInvalidDevice;

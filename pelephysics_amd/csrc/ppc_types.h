@@ -23,6 +23,10 @@ struct ReactParamsDev {
   BDFParamsDev bdf;
   RK64ParamsDev rk64;
   AdaptParamsDev adapt;
+  // per-lane matrix workspace of the larger mechanisms (lane_ws.h): ws_lanes
+  // lanes x the integrator's rows, or null
+  double* ws;
+  long long ws_lanes;
 };
 
 }  // namespace ppc

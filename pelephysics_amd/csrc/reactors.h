@@ -10,6 +10,7 @@
 //    Newton with a dense FP64 LU and Jacobian reuse.
 #pragma once
 #include "chem_cell.h"
+#include "lane_ws.h"
 #include "reactors_fwd.h"
 
 namespace ppc {
@@ -33,9 +34,8 @@ PPC_DI double fcoeff(int s, int m) {
 
 // Preconditioner solve M^-1 X (ReactorBDFsolver.H:12-91): 0 none, 1 Jacobi,
 // 2 symmetric Gauss-Seidel (D+L) D^-1 (D+U).
-template <int n>
-PPC_DI void precond(double* __restrict__ out, const double* __restrict__ A,
-                    const double* __restrict__ X, int type) {
+template <int n, class AM>
+PPC_DI void precond(double* __restrict__ out, const AM A, const double* __restrict__ X, int type) {
   if (type == 0) {
     for (int i = 0; i < n; ++i) out[i] = X[i];
   } else if (type == 1) {
@@ -58,13 +58,12 @@ PPC_DI void precond(double* __restrict__ out, const double* __restrict__ A,
 
 // Restarted GMRES with warm start (ReactorBDFsolver.H:384-499): solves A x = b
 // for the left-preconditioned system; x0 is the initial guess (overwritten).
-// Returns false when a NaN/Inf appears in the Hessenberg matrix.
-template <int n>
-PPC_DI bool gmres(const double* __restrict__ A, const double* __restrict__ b, double* __restrict__ x0,
+// V ((n+1) x n Krylov basis) and H ((n+1) x n Hessenberg matrix) are
+// caller-provided storage.  Returns false when a NaN/Inf appears in H.
+template <int n, class AM, class VM>
+PPC_DI bool gmres(const AM A, const VM V, const VM H, const double* __restrict__ b, double* __restrict__ x0,
                   double* __restrict__ x, int ptype, int restarts, int kspiters, double tol) {
   constexpr int K = n;
-  double V[(K + 1) * n];
-  double H[(K + 1) * K];
   double cs[K], sn[K], be[K + 1], yv[K + 1];
   double r[n], t1[n], t2[n];
   for (int i = 0; i < (K + 1) * K; ++i) H[i] = 0.0;
@@ -89,10 +88,9 @@ PPC_DI bool gmres(const double* __restrict__ A, const double* __restrict__ b, do
     int opt = K;
     for (int k = 0; k < kspiters; ++k) {
       for (int i = 0; i < n; ++i) x[i] = x0[i];
-      const double* vj = V + k * n;
       for (int i = 0; i < n; ++i) {
         double s = 0.0;
-        for (int j = 0; j < n; ++j) s += A[i * n + j] * vj[j];
+        for (int j = 0; j < n; ++j) s += A[i * n + j] * V[k * n + j];
         t1[i] = s;
       }
       precond<n>(t2, A, t1, ptype);
@@ -159,10 +157,14 @@ PPC_DI bool gmres(const double* __restrict__ A, const double* __restrict__ b, do
 template <class M>
 PPC_DI int bdf_parity_cell(const BDFParamsDev& p, const int reactor_type, double* __restrict__ y,
                            const double rhoe_init, const double rhoesrc, const double* __restrict__ rYsrc,
-                           const double dt_react) {
+                           const double dt_react, const LaneWS& w) {
   constexpr int n = M::NEQ;
+  constexpr bool WS = uses_ws<M>();
   double soln[n], soln_n[n], nm1[n], nm2[n], ds[n], ds0[n], rhs[n], f[n], fn[n];
-  double A[n * n];
+  double A_p[WS ? 1 : n * n], V_p[WS ? 1 : (n + 1) * n], H_p[WS ? 1 : (n + 1) * n];
+  const auto A = lane_array<WS>(A_p, w, 0);
+  const auto V = lane_array<WS>(V_p, w, n * n);
+  const auto H = lane_array<WS>(H_p, w, n * n + (n + 1) * n);
   const double dt = dt_react / (double)p.nsubsteps;
   const double dt_inv = 1.0 / dt;
   double current_time = 0.0;
@@ -196,7 +198,7 @@ PPC_DI int bdf_parity_cell(const BDFParamsDev& p, const int reactor_type, double
         r += f1 * fn[i];
         rhs[i] = r;
       }
-      if (!gmres<n>(A, rhs, ds0, ds, p.gmres_precond, p.gmres_restarts, p.gmres_kspiters, p.gmres_tol))
+      if (!gmres<n>(A, V, H, rhs, ds0, ds, p.gmres_precond, p.gmres_restarts, p.gmres_kspiters, p.gmres_tol))
         return -1;
       for (int i = 0; i < n; ++i) soln[i] += ds[i];
       double nrm = 0.0;
@@ -296,9 +298,18 @@ PPC_DI double rms_norm(const double* __restrict__ x, const double* __restrict__ 
   return sqrt(s / n);
 }
 
-// In-place LU with partial pivoting (row-major).
-template <int n, typename T>
-PPC_HEAVY void lu_factor(T* __restrict__ a, int* __restrict__ piv) {
+// In-place LU with partial pivoting (row-major); a is a T* or a LaneArr<T>.
+// The compiler cannot tell that the rows read and the row written differ, so
+// an element-by-element update would wait for each store before the next
+// load; the updates therefore go in chunks of LU_CHUNK columns whose loads are
+// all issued before their stores, with a chunk of the pivot row held in
+// registers across the rows below it.  Each element receives the same
+// operations in the same order as in the textbook k-i-j loop.
+constexpr int LU_CHUNK = 8;
+
+template <int n, typename T, class AM>
+PPC_HEAVY void lu_factor(const AM a, int* __restrict__ piv) {
+  constexpr int U = LU_CHUNK;
   for (int k = 0; k < n; ++k) {
     int p = k;
     T mx = fabs(a[k * n + k]);
@@ -311,22 +322,51 @@ PPC_HEAVY void lu_factor(T* __restrict__ a, int* __restrict__ piv) {
     }
     piv[k] = p;
     if (p != k)
-      for (int j = 0; j < n; ++j) {
-        const T t = a[k * n + j];
-        a[k * n + j] = a[p * n + j];
-        a[p * n + j] = t;
+      for (int j0 = 0; j0 < n; j0 += U) {
+        T rk[U], rp[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (j0 + u < n) {
+            rk[u] = a[k * n + j0 + u];
+            rp[u] = a[p * n + j0 + u];
+          }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (j0 + u < n) {
+            a[k * n + j0 + u] = rp[u];
+            a[p * n + j0 + u] = rk[u];
+          }
       }
     const T inv = T(1) / a[k * n + k];
-    for (int i = k + 1; i < n; ++i) {
-      const T l = a[i * n + k] * inv;
-      a[i * n + k] = l;
-      for (int j = k + 1; j < n; ++j) a[i * n + j] -= l * a[k * n + j];
+    for (int i0 = k + 1; i0 < n; i0 += U) {  // multipliers l_ik = a_ik / a_kk
+      T l[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (i0 + u < n) l[u] = a[(i0 + u) * n + k];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (i0 + u < n) a[(i0 + u) * n + k] = l[u] * inv;
+    }
+    for (int j0 = k + 1; j0 < n; j0 += U) {
+      T rk[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) rk[u] = (j0 + u < n) ? a[k * n + j0 + u] : T(0);
+      for (int i = k + 1; i < n; ++i) {
+        const T l = a[i * n + k];
+        T ri[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (j0 + u < n) ri[u] = a[i * n + j0 + u];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (j0 + u < n) a[i * n + j0 + u] = ri[u] - l * rk[u];
+      }
     }
   }
 }
 
-template <int n, typename T>
-PPC_HEAVY void lu_solve(const T* __restrict__ a, const int* __restrict__ piv, double* __restrict__ b) {
+template <int n, typename T, class AM>
+PPC_HEAVY void lu_solve(const AM a, const int* __restrict__ piv, double* __restrict__ b) {
   for (int k = 0; k < n; ++k)
     if (piv[k] != k) {
       const double t = b[k];
@@ -384,13 +424,16 @@ PPC_HEAVY void change_D(double* __restrict__ D, const int order, const double fa
 template <class M, typename LU_T>
 PPC_DI int adaptive_cell(const AdaptParamsDev& p, const int reactor_type, double* __restrict__ y0,
                          const double rhoe_init, const double rhoesrc, const double* __restrict__ rYsrc,
-                         const double t_bound, int* __restrict__ cnt) {
+                         const double t_bound, int* __restrict__ cnt, const LaneWS& w) {
   constexpr int n = M::NEQ;
+  constexpr bool WS = uses_ws<M>();
   const double rtol = p.rtol, atol = p.atol;
   const double newton_tol = fmax(10 * 2.220446049250313e-16 / rtol, fmin(0.03, sqrt(rtol)));
   double D[(AMAXORD + 3) * n];
-  double J[n * n];
-  LU_T LU[n * n];
+  double J_p[WS ? 1 : n * n];
+  LU_T LU_p[WS ? 1 : n * n];
+  const auto J = lane_array<WS>(J_p, w, 0);
+  const auto LU = lane_array<WS>(LU_p, w, n * n);
   int piv[n];
   double ypred[n], scale[n], psi[n], yv[n], d[n], f[n], dy[n];
   int nsteps = 0, nrhs = 0, njac = 0, nlu = 0, nnewton = 0, nrej = 0;
@@ -465,7 +508,15 @@ PPC_DI int adaptive_cell(const AdaptParamsDev& p, const int reactor_type, double
       const double c = h * inv_alpha;
       while (!converged) {
         if (!lu_valid) {
-          for (int i = 0; i < n * n; ++i) LU[i] = (LU_T)(-c * J[i]);
+          for (int i0 = 0; i0 < n * n; i0 += LU_CHUNK) {  // loads of a chunk before its stores
+            double v[LU_CHUNK];
+#pragma unroll
+            for (int u = 0; u < LU_CHUNK; ++u)
+              if (i0 + u < n * n) v[u] = J[i0 + u];
+#pragma unroll
+            for (int u = 0; u < LU_CHUNK; ++u)
+              if (i0 + u < n * n) LU[i0 + u] = (LU_T)(-c * v[u]);
+          }
           for (int i = 0; i < n; ++i) LU[i * n + i] += (LU_T)1;
           lu_factor<n, LU_T>(LU, piv);
           nlu++;
@@ -581,10 +632,10 @@ PPC_DI int adaptive_cell(const AdaptParamsDev& p, const int reactor_type, double
 // -------------------------------------------------------------- kernels --
 template <class M, int MODE>
 PPC_DI int integrate_cell(const ReactParamsDev& P, double* y, double rhoe, double rhoesrc,
-                          const double* src, double dt, int* cnt) {
-  if (MODE == MODE_BDF) return bdf_parity_cell<M>(P.bdf, P.reactor_type, y, rhoe, rhoesrc, src, dt);
+                          const double* src, double dt, int* cnt, const LaneWS& w) {
+  if (MODE == MODE_BDF) return bdf_parity_cell<M>(P.bdf, P.reactor_type, y, rhoe, rhoesrc, src, dt, w);
   if (MODE == MODE_RK64) return rk64_cell<M>(P.rk64, P.reactor_type, y, rhoe, rhoesrc, src, dt);
-  return adaptive_cell<M, double>(P.adapt, P.reactor_type, y, rhoe, rhoesrc, src, dt, cnt);
+  return adaptive_cell<M, double>(P.adapt, P.reactor_type, y, rhoe, rhoesrc, src, dt, cnt, w);
 }
 
 PPC_DI void store_counters(const CellStats& st, int c, int nsteps, const int* cnt) {
@@ -598,25 +649,29 @@ PPC_DI void store_counters(const CellStats& st, int c, int nsteps, const int* cn
   }
 }
 
-// 1-D (YCOrder) form: ReactorBDF.cpp:133-311 / ReactorRK64.cpp:22-164.
+// 1-D (YCOrder) form: ReactorBDF.cpp:133-311 / ReactorRK64.cpp:22-164.  One
+// lane per cell; with a workspace (lane_ws.h) the grid is what the workspace
+// holds and lanes stride over the cells.
 template <class M, int MODE>
 __global__ void __launch_bounds__(64) react_1d_kernel(ReactParamsDev P, double* __restrict__ yv,
                                                       const double* __restrict__ ysrc,
                                                       double* __restrict__ rX,
                                                       const double* __restrict__ rXsrc, double dt,
                                                       int ncells, CellStats st) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const LaneWS w{P.ws, P.ws_lanes, g};
   constexpr int n = M::NEQ;
-  double y[n], src[M::NS];
-  for (int k = 0; k < n; ++k) y[k] = yv[(long long)c * n + k];
-  for (int k = 0; k < M::NS; ++k) src[k] = ysrc[(long long)c * M::NS + k];
-  const double rhoe = rX[c], rhoesrc = rXsrc[c];
-  int cnt[5] = {0, 0, 0, 0, 0};
-  const int r = integrate_cell<M, MODE>(P, y, rhoe, rhoesrc, src, dt, cnt);
-  for (int k = 0; k < n; ++k) yv[(long long)c * n + k] = y[k];
-  rX[c] = rhoe + dt * rhoesrc;
-  store_counters(st, c, r, cnt);
+  for (int c = g; c < ncells; c += gridDim.x * blockDim.x) {
+    double y[n], src[M::NS];
+    for (int k = 0; k < n; ++k) y[k] = yv[(long long)c * n + k];
+    for (int k = 0; k < M::NS; ++k) src[k] = ysrc[(long long)c * M::NS + k];
+    const double rhoe = rX[c], rhoesrc = rXsrc[c];
+    int cnt[5] = {0, 0, 0, 0, 0};
+    const int r = integrate_cell<M, MODE>(P, y, rhoe, rhoesrc, src, dt, cnt, w);
+    for (int k = 0; k < n; ++k) yv[(long long)c * n + k] = y[k];
+    rX[c] = rhoe + dt * rhoesrc;
+    store_counters(st, c, r, cnt);
+  }
 }
 
 // Box (Array4) form: ReactorBDF.cpp:314-494 / ReactorRK64.cpp:166-320:
@@ -625,30 +680,32 @@ __global__ void __launch_bounds__(64) react_1d_kernel(ReactParamsDev P, double* 
 template <class M, int MODE>
 __global__ void __launch_bounds__(64) react_box_kernel(ReactParamsDev P, BoxArgs b, double dt,
                                                        int ncells, CellStats st) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
-  const int i = b.lo[0] + c % b.len[0];
-  const int j = b.lo[1] + (c / b.len[0]) % b.len[1];
-  const int k = b.lo[2] + c / (b.len[0] * b.len[1]);
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const LaneWS w{P.ws, P.ws_lanes, g};
   constexpr int NS = M::NS, n = M::NEQ;
-  double y[n], src[NS], Y[NS];
-  for (int s = 0; s < NS; ++s) {
-    y[s] = b.rY(i, j, k, s);
-    src[s] = b.rYsrc(i, j, k, s);
+  for (int c = g; c < ncells; c += gridDim.x * blockDim.x) {
+    const int i = b.lo[0] + c % b.len[0];
+    const int j = b.lo[1] + (c / b.len[0]) % b.len[1];
+    const int k = b.lo[2] + c / (b.len[0] * b.len[1]);
+    double y[n], src[NS], Y[NS];
+    for (int s = 0; s < NS; ++s) {
+      y[s] = b.rY(i, j, k, s);
+      src[s] = b.rYsrc(i, j, k, s);
+    }
+    double rinv;
+    double rho = rho_and_Y<M>(y, Y, rinv);
+    const double rhoe = b.rE(i, j, k, 0), rhoesrc = b.rEsrc(i, j, k, 0);
+    y[NS] = T_from_energy_rt<M>(P.reactor_type, rhoe / rho, Y, b.T(i, j, k, 0));
+    int cnt[5] = {0, 0, 0, 0, 0};
+    const int r = integrate_cell<M, MODE>(P, y, rhoe, rhoesrc, src, dt, cnt, w);
+    rho = rho_and_Y<M>(y, Y, rinv);
+    for (int s = 0; s < NS; ++s) b.rY(i, j, k, s) = y[s];
+    const double rE_new = rhoe + dt * rhoesrc;
+    b.rE(i, j, k, 0) = rE_new;
+    b.T(i, j, k, 0) = T_from_energy_rt<M>(P.reactor_type, rE_new / rho, Y, y[NS]);
+    if (b.FC.p) b.FC(i, j, k, 0) = (double)r;
+    store_counters(st, c, r, cnt);
   }
-  double rinv;
-  double rho = rho_and_Y<M>(y, Y, rinv);
-  const double rhoe = b.rE(i, j, k, 0), rhoesrc = b.rEsrc(i, j, k, 0);
-  y[NS] = T_from_energy_rt<M>(P.reactor_type, rhoe / rho, Y, b.T(i, j, k, 0));
-  int cnt[5] = {0, 0, 0, 0, 0};
-  const int r = integrate_cell<M, MODE>(P, y, rhoe, rhoesrc, src, dt, cnt);
-  rho = rho_and_Y<M>(y, Y, rinv);
-  for (int s = 0; s < NS; ++s) b.rY(i, j, k, s) = y[s];
-  const double rE_new = rhoe + dt * rhoesrc;
-  b.rE(i, j, k, 0) = rE_new;
-  b.T(i, j, k, 0) = T_from_energy_rt<M>(P.reactor_type, rE_new / rho, Y, y[NS]);
-  if (b.FC.p) b.FC(i, j, k, 0) = (double)r;
-  store_counters(st, c, r, cnt);
 }
 
 }  // namespace ppc

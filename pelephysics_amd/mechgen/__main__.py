@@ -4,7 +4,8 @@
 
 Writes ``pelephysics_amd/mechanisms/<mech>.json`` (normalised parameter
 tables, read by the oracle and the tests) and
-``pelephysics_amd/csrc/mech/<mech>.h`` (unrolled device code) and
+``pelephysics_amd/csrc/mech/<mech>.h`` (unrolled device code, small
+mechanisms only) and
 ``pelephysics_amd/csrc/tab/<mech>.h`` (constant-memory tables).
 """
 import argparse
@@ -16,7 +17,10 @@ from .emit_tables import emit as emit_tables
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
-DEFAULT_MECHS = ["LiDryer", "drm19"]
+DEFAULT_MECHS = ["LiDryer", "drm19", "dodecane_lu", "heptane_lu_88sk"]
+# mechanisms that also get the fully unrolled device code (mech/<m>.h); the
+# larger ones are compiled from the constant-memory tables only
+UNROLLED = ("LiDryer", "drm19")
 
 
 def generate(mechs, ref="/root/reference"):
@@ -26,8 +30,9 @@ def generate(mechs, ref="/root/reference"):
         m = load_yaml(src, name)
         with open(os.path.join(PKG, "mechanisms", f"{name}.json"), "w") as f:
             f.write(m.to_json())
-        with open(os.path.join(PKG, "csrc", "mech", f"{name}.h"), "w") as f:
-            f.write(emit(m, f"Mechanisms/{name}/mechanism.yaml"))
+        if name in UNROLLED:
+            with open(os.path.join(PKG, "csrc", "mech", f"{name}.h"), "w") as f:
+                f.write(emit(m, f"Mechanisms/{name}/mechanism.yaml"))
         with open(os.path.join(PKG, "csrc", "tab", f"{name}.h"), "w") as f:
             f.write(emit_tables(m, f"Mechanisms/{name}/mechanism.yaml"))
         out.append(name)

@@ -120,8 +120,13 @@ class Mechanism:
         return len(self.reactions)
 
     def species_index(self, name: str) -> int:
+        """Index of a species by name; case-insensitive when there is no exact
+        match (heptane_lu_88sk spells its species in lower case)."""
         for i, s in enumerate(self.species):
             if s.name == name:
+                return i
+        for i, s in enumerate(self.species):
+            if s.name.lower() == name.lower():
                 return i
         raise KeyError(name)
 

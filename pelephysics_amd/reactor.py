@@ -63,7 +63,7 @@ class Config(C.Structure):
         ("rk64_atol", C.c_double), ("rk64_nsubsteps_guess", C.c_int),
         ("rk64_nsubsteps_min", C.c_int), ("rk64_nsubsteps_max", C.c_int),
         ("rtol", C.c_double), ("atol", C.c_double), ("first_step", C.c_double),
-        ("max_steps", C.c_int), ("block_size", C.c_int),
+        ("max_steps", C.c_int), ("block_size", C.c_int), ("workspace_bytes", C.c_longlong),
     ]
 
 
@@ -93,7 +93,7 @@ _KEYS = {
     "rtol": "rtol", "first_step": "first_step", "max_steps": "max_steps",
     "rk64_nsubsteps_guess": "rk64_nsubsteps_guess", "rk64_nsubsteps_min": "rk64_nsubsteps_min",
     "rk64_nsubsteps_max": "rk64_nsubsteps_max", "mod_reactor": "mod_reactor",
-    "block_size": "block_size",
+    "block_size": "block_size", "workspace_bytes": "workspace_bytes",
 }
 
 _lib = None
@@ -112,6 +112,8 @@ def library() -> C.CDLL:
         L.pp_chem_destroy.argtypes = [vp]
         L.pp_chem_num_species.argtypes = [vp]
         L.pp_chem_num_reactions.argtypes = [vp]
+        L.pp_chem_workspace_rows.argtypes = [vp]
+        L.pp_chem_workspace_rows.restype = C.c_longlong
         L.pp_chem_adjoint_1d.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, vp, vp,
                                          vp]
         L.pp_chem_list_mechanisms.argtypes = [C.c_char_p, C.c_int]
@@ -278,6 +280,10 @@ class ReactorBase:
         s = Stats()
         library().pp_chem_get_stats(self._h, C.byref(s))
         return s.as_dict()
+
+    def workspace_rows(self) -> int:
+        """Doubles per resident cell of the per-cell matrix workspace (0: none)."""
+        return int(library().pp_chem_workspace_rows(self._h))
 
     def cell_steps(self, ncells: int) -> np.ndarray:
         out = np.zeros(ncells, np.int32)

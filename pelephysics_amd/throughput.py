@@ -1,0 +1,92 @@
+"""Throughput of one integrator on a synthetic workload, state resident on the device.
+
+    python -m pelephysics_amd.throughput --mech heptane_lu_88sk --cells 131072 \\
+        --integrator ReactorBDFAdaptiveMixed --calls 3 [--chained] [--rtol 1e-10 --atol 1e-16]
+
+Workloads (pelephysics_amd.workloads): LiDryer -> lidryer_h2air, drm19 ->
+drm19_grf (cells rounded to a cube), dodecane_lu -> dodecane_uneven,
+heptane_lu_88sk -> heptane_ignition.  Each call integrates dt = 1e-5 s with
+the 1-D form on device pointers; without --chained every call restarts from
+the initial state, with --chained the calls continue from each other (as a
+flow solver's successive chemistry steps do).  Prints one JSON line: cells/s
+from the kernel time of each call (HIP events around the launch), the
+per-cell work counters and the matrix workspace per resident cell.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import workloads as W
+from .mechgen.__main__ import load
+from .reactor import ReactorBase
+
+GENERATORS = {"LiDryer": W.lidryer_h2air, "dodecane_lu": W.dodecane_uneven,
+              "heptane_lu_88sk": W.heptane_ignition}
+
+
+def workload(mech: str, ncells: int) -> W.CellState:
+    m = load(mech)
+    if mech == "drm19":
+        side = max(1, round(ncells ** (1.0 / 3.0)))
+        return W.drm19_grf(m, side)
+    return GENERATORS[mech](m, ncells)
+
+
+def main(argv=None) -> dict:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--mech", default="heptane_lu_88sk")
+    ap.add_argument("--cells", type=int, default=131072)
+    ap.add_argument("--integrator", default="ReactorBDFAdaptiveMixed")
+    ap.add_argument("--calls", type=int, default=3)
+    ap.add_argument("--chained", action="store_true")
+    ap.add_argument("--dt", type=float, default=1e-5)
+    ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--atol", type=float, default=1e-16)
+    a = ap.parse_args(argv)
+
+    import torch
+    st = workload(a.mech, a.cells)
+    m = load(a.mech)
+    dev = torch.device("cuda:0")
+    R = ReactorBase.create(a.integrator)
+    R.init(2, st.ncells, mechanism=a.mech, rtol=a.rtol, atol=a.atol)
+    y0 = torch.from_numpy(st.y_1d()).to(dev)
+    x0 = torch.from_numpy(st.rE.copy()).to(dev)
+    y, x = y0.clone(), x0.clone()
+    ys = torch.zeros(st.ncells * m.n_species, dtype=torch.float64, device=dev)
+    xs = torch.zeros(st.ncells, dtype=torch.float64, device=dev)
+    kms, per_call = [], []
+    t0 = time.perf_counter()
+    for call in range(a.calls):
+        if not a.chained:
+            y.copy_(y0)
+            x.copy_(x0)
+        R.react_1d(y, ys, x, xs, a.dt)
+        s = R.stats()
+        kms.append(s["kernel_ms"])
+        per_call.append({k: s[k] / max(1, s["ncells"]) for k in ("nsteps", "nrhs", "njac", "nlu")})
+        print(f"[throughput] call {call + 1}/{a.calls}: kernel {s['kernel_ms']:.1f} ms, "
+              f"{s['nsteps'] / max(1, s['ncells']):.1f} steps/cell, failed {s['nfailed']}",
+              file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = {
+        "mech": a.mech, "integrator": a.integrator, "cells": st.ncells, "dt": a.dt,
+        "rtol": a.rtol, "atol": a.atol, "calls": a.calls, "chained": a.chained,
+        "cells_per_s": st.ncells / (float(np.median(kms)) * 1e-3),
+        "kernel_ms": kms, "wall_s": wall, "per_cell": per_call,
+        "workspace_bytes_per_resident_cell": R.workspace_rows() * 8,
+        "finite": bool(torch.isfinite(y).all().item()),
+    }
+    R.close()
+    print(json.dumps(out))
+    return out
+
+
+if __name__ == "__main__":
+    main()

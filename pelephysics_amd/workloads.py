@@ -11,6 +11,13 @@ numpy arrays: ``rY`` (ncells, N) in g/cm^3, ``T`` (ncells,) in K, ``rE``
   c ~ U[0,1] interpolating linearly between fresh gas at 300 K and complete
   combustion products at 2200 K (64^3 cells by default).
 * ``drm19_fresh``: CH4/air, T ~ U[1100,1500] K, phi ~ U[0.6,1.4], p = 1 atm.
+* ``dodecane_uneven``: n-C12H26/air cells of very uneven stiffness, shuffled:
+  70% cold fresh gas (T ~ U[300,700] K, phi = 1, p = 1 atm), 25% near
+  ignition (T ~ U[900,1200] K, phi ~ U[0.7,1.3], p ~ U[1,40] atm), 5% complete
+  combustion products at 2300 K (2^20 cells by default).
+* ``heptane_ignition``: n-C7H16/air near ignition, T ~ U[800,1300] K,
+  phi ~ U[0.5,2.0], p ~ U[10,50] atm (131072 cells by default).
+* ``premixed``: a small fuel/air state for any compiled mechanism (tests).
 
 Seeds are fixed (numpy PCG64) so every run sees the same cells.
 """
@@ -152,3 +159,65 @@ def drm19_fresh(m: Mechanism, ncells: int = 4096, seed: int = SEED,
 def subsample(st: CellState, idx: np.ndarray) -> CellState:
     return CellState(st.mech, np.ascontiguousarray(st.rY[idx]), np.ascontiguousarray(st.T[idx]),
                      np.ascontiguousarray(st.rE[idx]), st.p[idx])
+
+
+# Fuel of each mechanism and its stoichiometric O2 per fuel molecule.
+FUELS = {"LiDryer": ("H2", 0.5), "drm19": ("CH4", 2.0), "dodecane_lu": ("NC12H26", 18.5),
+         "heptane_lu_88sk": ("NC7H16", 11.0)}
+
+
+def _fuel_air_Y(m: Mechanism, phi: np.ndarray) -> np.ndarray:
+    fuel, s = FUELS[m.name]
+    n = phi.shape[0]
+    return _mass_from_moles(m, {fuel: phi, "O2": s * np.ones(n), "N2": 3.76 * s * np.ones(n)}, n)
+
+
+def _products_Y(m: Mechanism, phi: np.ndarray) -> np.ndarray:
+    """Complete combustion products (CO2, H2O, N2 and the excess O2 or fuel)."""
+    fuel, s = FUELS[m.name]
+    comp = m.species[m.species_index(fuel)].composition
+    nC, nH = comp.get("C", 0.0), comp.get("H", 0.0)
+    n = phi.shape[0]
+    burnt = np.minimum(phi, 1.0)  # moles of fuel burnt per s moles of O2
+    X = {fuel: phi - burnt, "O2": s * (1.0 - burnt), "N2": 3.76 * s * np.ones(n),
+         "H2O": 0.5 * nH * burnt}
+    if nC > 0:
+        X["CO2"] = nC * burnt
+    return _mass_from_moles(m, X, n)
+
+
+def dodecane_uneven(m: Mechanism, ncells: int = 2 ** 20, seed: int = SEED) -> CellState:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n_fresh, n_ign = int(0.70 * ncells), int(0.25 * ncells)
+    n_burnt = ncells - n_fresh - n_ign
+    T = np.concatenate([rng.uniform(300.0, 700.0, n_fresh), rng.uniform(900.0, 1200.0, n_ign),
+                        np.full(n_burnt, 2300.0)])
+    phi = np.concatenate([np.ones(n_fresh), rng.uniform(0.7, 1.3, n_ign), np.ones(n_burnt)])
+    p = np.concatenate([np.full(n_fresh, PATM), rng.uniform(1.0, 40.0, n_ign) * PATM, np.full(n_burnt, PATM)])
+    Y = np.concatenate([_fuel_air_Y(m, phi[:n_fresh + n_ign]), _products_Y(m, phi[n_fresh + n_ign:])])
+    perm = rng.permutation(ncells)
+    return _finish(m, m.name, Y[perm], T[perm], p[perm])
+
+
+def heptane_ignition(m: Mechanism, ncells: int = 131072, seed: int = SEED) -> CellState:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    T = rng.uniform(800.0, 1300.0, ncells)
+    phi = rng.uniform(0.5, 2.0, ncells)
+    p = rng.uniform(10.0, 50.0, ncells) * PATM
+    return _finish(m, m.name, _fuel_air_Y(m, phi), T, p)
+
+
+def premixed(m: Mechanism, ncells: int, seed: int = SEED) -> CellState:
+    """Small fuel/air state of any compiled mechanism: LiDryer and drm19 as
+    their workloads above, the larger mechanisms near ignition."""
+    if m.name == "LiDryer":
+        return lidryer_h2air(m, ncells, seed)
+    if m.name == "drm19":
+        return drm19_fresh(m, ncells, seed)
+    if m.name == "dodecane_lu":
+        rng = np.random.Generator(np.random.PCG64(seed))
+        T = rng.uniform(900.0, 1200.0, ncells)
+        phi = rng.uniform(0.7, 1.3, ncells)
+        p = rng.uniform(1.0, 40.0, ncells) * PATM
+        return _finish(m, m.name, _fuel_air_Y(m, phi), T, p)
+    return heptane_ignition(m, ncells, seed)

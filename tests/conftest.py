@@ -15,7 +15,7 @@ def pytest_configure(config):
 @pytest.fixture(scope="session")
 def models():
     from pelephysics_amd.mechgen.__main__ import load
-    return {m: load(m) for m in ("LiDryer", "drm19")}
+    return {m: load(m) for m in ("LiDryer", "drm19", "dodecane_lu", "heptane_lu_88sk")}
 
 
 @pytest.fixture(scope="session")

@@ -156,10 +156,12 @@ def extract(mech: str) -> dict:
         r["reactants"] = reac
         mq = re.search(r"qr = (.*?);", flat)
         qr = mq.group(1) if mq else "0.0"
-        r["reversible"] = "g_RT" in qr
+        env = {"Corr": 1.0, "k_f": 1.0, "g_RT": [0.0] * nsp, "sc": [1.0] * nsp,
+               "refC": 2.0, "refCinv": 0.5}
+        # some irreversible reactions are emitted with a reverse rate times a
+        # literal 0.0 (e.g. heptane_lu_88sk reaction 21)
+        r["reversible"] = "g_RT" in qr and _eval(qr, env) != 0.0
         if r["reversible"]:
-            env = {"Corr": 1.0, "k_f": 1.0, "g_RT": [0.0] * nsp, "sc": [1.0] * nsp,
-                   "refC": 2.0, "refCinv": 0.5}
             r["refc_power"] = math.log2(_eval(qr, env))
         else:
             r["refc_power"] = None

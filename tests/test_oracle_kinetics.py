@@ -22,7 +22,7 @@ import numpy as np
 import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-MECHS = ["LiDryer", "drm19"]
+MECHS = ["LiDryer", "drm19", "dodecane_lu", "heptane_lu_88sk"]
 
 
 def _fix(mech):
