@@ -58,6 +58,9 @@ public:
     pp.query("rtol", m_cfg.rtol);
     pp.query("atol", m_cfg.atol);
     pp.query("max_steps", m_cfg.max_steps);
+    bool clean = false;
+    pp.query("clean_init_massfrac", clean);  // ReactorBDF.cpp:127, ReactorRK64.cpp:18
+    m_cfg.clean_init_massfrac = clean ? 1 : 0;
 #ifdef MOD_REACTOR
     m_cfg.mod_reactor = 1;
 #endif
@@ -158,9 +161,17 @@ public:
     amrex::Real* vect_energy,
     amrex::Real* src_vect_energy) override
   {
-    m_flatten.flatten(
-      box, ncells, m_cfg.reactor_type, false, rhoY, frcExt, temperature, rhoE,
-      frcEExt, y_vect, src_vect, vect_energy, src_vect_energy);
+    AMREX_ALWAYS_ASSERT(ncells == box.numPts());
+    const auto lo = amrex::lbound(box);
+    const auto hi = amrex::ubound(box);
+    const int l[3] = {lo.x, lo.y, lo.z}, h[3] = {hi.x, hi.y, hi.z};
+    const pp_array4 a[5] = {view(rhoY), view(frcExt), view(temperature), view(rhoE), view(frcEExt)};
+    const int r = pp_chem_flatten(
+      m_h, l, h, &a[0], &a[1], &a[2], &a[3], &a[4], y_vect, src_vect, vect_energy,
+      src_vect_energy, current_stream());
+    if (r < 0) {
+      amrex::Abort(std::string("ReactorHIP: ") + pp_chem_last_error(m_h));
+    }
   }
 
   void unflatten(
@@ -176,14 +187,33 @@ public:
     long int* FCunt,
     amrex::Real dt) override
   {
-    m_flatten.unflatten(
-      box, ncells, m_cfg.reactor_type, false, rhoY, temperature, rhoE, frcEExt,
-      FC_in, y_vect, vect_energy, FCunt, dt);
+    static_assert(sizeof(long int) == sizeof(long long), "FCunt is read as 64-bit counts");
+    AMREX_ALWAYS_ASSERT(ncells == box.numPts());
+    const auto lo = amrex::lbound(box);
+    const auto hi = amrex::ubound(box);
+    const int l[3] = {lo.x, lo.y, lo.z}, h[3] = {hi.x, hi.y, hi.z};
+    const pp_array4 a[5] = {view(rhoY), view(temperature), view(rhoE), view(frcEExt), view(FC_in)};
+    const int r = pp_chem_unflatten(
+      m_h, l, h, &a[0], &a[1], &a[2], &a[3], &a[4], y_vect, vect_energy,
+      reinterpret_cast<const long long*>(FCunt), dt, current_stream());
+    if (r < 0) {
+      amrex::Abort(std::string("ReactorHIP: ") + pp_chem_last_error(m_h));
+    }
   }
 
   ~ReactorHIP() override { close(); }
 
 private:
+  // flatten/unflatten carry no stream argument: AMReX's current stream
+  static void* current_stream()
+  {
+#ifdef AMREX_USE_GPU
+    return (void*)amrex::Gpu::gpuStream();
+#else
+    return nullptr;
+#endif
+  }
+
   template <class T>
   static pp_array4 view(amrex::Array4<T> const& a)
   {
@@ -205,7 +235,6 @@ private:
   pp_chem_config m_cfg{};
   std::string m_mech;
   pp_chem* m_h = nullptr;
-  utils::FlattenOps<utils::YCOrder> m_flatten;
 };
 
 inline constexpr char kReactorBDF_HIP[] = "ReactorBDF_HIP";

@@ -70,6 +70,10 @@ typedef struct {
    * for a lane's private memory); 0: default (32 GiB).  Fewer resident cells
    * are integrated at once when it is reached. */
   long long workspace_bytes;
+  /* ode.clean_init_massfrac (ReactorBDF.cpp:127, ReactorRK64.cpp:18): the
+   * box <-> flat conversion clips rhoY to [0, rho] and re-applies the
+   * increment conservatively (pp_chem_flatten / pp_chem_unflatten). */
+  int clean_init_massfrac;
 } pp_chem_config;
 
 /* amrex::Array4<Real> layout: element (i,j,k,n) is at
@@ -111,6 +115,24 @@ int pp_chem_react_box(pp_chem* h, const int lo[3], const int hi[3], const pp_arr
  * GPU builds, ReactorBDF.cpp:155-171,295-302); 1: device pointers. */
 int pp_chem_react_1d(pp_chem* h, double* y, double* ysrc, double* rX, double* rXsrc,
                      double* dt_react, double* time, int ncells, int on_device, void* stream);
+
+/* Box -> flat state, ReactorBase::flatten (ReactorBase.H:56-67; FlattenOps<YCOrder>,
+ * ReactorUtils.H:160-220,291-325): y_vect (ncells x (N+1), T from the energy
+ * with the box T as guess), src_vect (ncells x N), vect_energy and
+ * src_vect_energy (ncells); all device pointers, cell index
+ * (k-lo2)*len0*len1 + (j-lo1)*len0 + (i-lo0).  With clean_init_massfrac the
+ * flat rhoY are clipped to [0, sum rhoY]. */
+int pp_chem_flatten(pp_chem* h, const int lo[3], const int hi[3], const pp_array4* rY, const pp_array4* rYsrc,
+                    const pp_array4* T, const pp_array4* rE, const pp_array4* rEsrc, double* y_vect,
+                    double* src_vect, double* vect_energy, double* src_vect_energy, void* stream);
+/* Flat -> box state, ReactorBase::unflatten (ReactorBase.H:69-80;
+ * ReactorUtils.H:222-289,327-355): rY from y_vect (with clean_init_massfrac:
+ * old rhoY plus the increment over its clipped value, scaled by
+ * rho_old / rho_clipped), rE = vect_energy + dt * rEsrc, T from the energy with
+ * the flat T as guess, FC = FCunt (the reference's long int counts). */
+int pp_chem_unflatten(pp_chem* h, const int lo[3], const int hi[3], const pp_array4* rY, const pp_array4* T,
+                      const pp_array4* rE, const pp_array4* rEsrc, const pp_array4* FC, const double* y_vect,
+                      const double* vect_energy, const long long* FCunt, double dt, void* stream);
 
 int pp_chem_get_stats(const pp_chem* h, pp_chem_stats* out);
 int pp_chem_num_reactions(const pp_chem* h);

@@ -286,6 +286,60 @@ class Mech:
                                         time, ncells, C.byref(st), nthreads)
         return r, st.as_dict()
 
+    # -- box <-> flat state (utils::box_flatten / box_unflatten, YCOrder)
+    def flatten(self, rY, rYsrc, T, rE, rEsrc, reactor_type=2, clean=False):
+        """Box cells (rY[cell, k], T/rE/rEsrc[cell]) -> (y_vect, src_vect,
+        vect_energy, src_vect_energy) as Source/Reactions/ReactorUtils.H:160-220:
+        rhoY clipped to [0, rho_old] with clean_init_massfrac (:180-196), T from
+        the energy per unit mass with the box T as guess (:207-219)."""
+        ncells, ns = rY.shape
+        y = np.zeros((ncells, ns + 1))
+        for c in range(ncells):
+            rho_old = 0.0
+            if clean:
+                for n in range(ns):
+                    rho_old += rY[c, n]
+            rho = 0.0
+            for n in range(ns):
+                v = min(max(0.0, rY[c, n]), rho_old) if clean else rY[c, n]
+                y[c, n] = v
+                rho += v
+            Y = y[c, :ns] * (1.0 / rho)
+            ev = rE[c] * (1.0 / rho)
+            inv = self.T_from_e if reactor_type == 1 else self.T_from_h
+            y[c, ns] = inv(ev, Y, T[c])[0]
+        return y.reshape(-1), np.array(rYsrc, float).reshape(-1), np.array(rE, float), np.array(rEsrc, float)
+
+    def unflatten(self, rY, T, rE, rEsrc, y_vect, vect_energy, FCunt, dt, reactor_type=2, clean=False):
+        """Flat -> box in place (rY[cell, k], T, rE[cell]); returns FC.  As
+        Source/Reactions/ReactorUtils.H:222-289: with clean_init_massfrac the
+        new rhoY is the old one plus the increment over its clipped value times
+        rho_old / rho_clipped (:245-271); rE = vect_energy + dt * rEsrc (:278);
+        T from the energy with the flat T as guess (:280-287)."""
+        ncells, ns = rY.shape
+        y = np.asarray(y_vect).reshape(ncells, ns + 1)
+        for c in range(ncells):
+            if clean:
+                rho_old = 0.0
+                for n in range(ns):
+                    rho_old += rY[c, n]
+                clip = [min(max(0.0, rY[c, n]), rho_old) for n in range(ns)]
+                rho_clip = 0.0
+                for n in range(ns):
+                    rho_clip += clip[n]
+                new = [rY[c, n] + (y[c, n] - clip[n]) * rho_old / rho_clip for n in range(ns)]
+            else:
+                new = [y[c, n] for n in range(ns)]
+            rho = 0.0
+            for n in range(ns):
+                rY[c, n] = new[n]
+                rho += new[n]
+            Y = rY[c] * (1.0 / rho)
+            rE[c] = vect_energy[c] + dt * rEsrc[c]
+            inv = self.T_from_e if reactor_type == 1 else self.T_from_h
+            T[c] = inv(rE[c] * (1.0 / rho), Y, y[c, ns])[0]
+        return np.asarray(FCunt, float).copy()
+
     def react_box(self, mode, rY, rYsrc, T, rE, rEsrc, dt, reactor_type=2, params=None,
                   nthreads=0, time=0.0):
         """Contiguous SoA box (rY[k, cell]); mode 'bdf' | 'rk64' | 'adaptive'."""

@@ -310,6 +310,39 @@ ppc::A4 to_a4(const pp_array4* a) {
   return r;
 }
 
+// Box geometry and views; every non-null view must cover the box and have
+// enough components.
+int box_args(pp_chem* h, const int lo[3], const int hi[3], const pp_array4* rY, const pp_array4* rYsrc,
+             const pp_array4* T, const pp_array4* rE, const pp_array4* rEsrc, const pp_array4* FC,
+             ppc::BoxArgs& b, int& ncells) {
+  long long nc = 1;
+  b = ppc::BoxArgs{};
+  for (int d = 0; d < 3; ++d) {
+    b.lo[d] = lo[d];
+    b.len[d] = hi[d] - lo[d] + 1;
+    if (b.len[d] < 0) return fail(h, PP_CHEM_EINVAL, "hi < lo");
+    nc *= b.len[d];
+  }
+  if (nc > 0x7fffffffLL) return fail(h, PP_CHEM_EINVAL, "box too large");
+  ncells = (int)nc;
+  const pp_array4* views[] = {rY, rYsrc, T, rE, rEsrc, FC};
+  const int need[] = {h->L->ns, h->L->ns, 1, 1, 1, 1};
+  for (int v = 0; v < 6; ++v) {
+    const pp_array4* a = views[v];
+    if (!a || !a->p) continue;
+    for (int d = 0; d < 3; ++d)
+      if (lo[d] < a->begin[d] || hi[d] >= a->end[d]) return fail(h, PP_CHEM_EINVAL, "Array4 does not cover the box");
+    if (a->ncomp < need[v]) return fail(h, PP_CHEM_EINVAL, "Array4 has too few components");
+  }
+  b.rY = to_a4(rY);
+  b.rYsrc = to_a4(rYsrc);
+  b.T = to_a4(T);
+  b.rE = to_a4(rE);
+  b.rEsrc = to_a4(rEsrc);
+  b.FC = to_a4(FC);
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -379,6 +412,7 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
       (!(cfg->rtol > 0) || !(cfg->atol > 0) || cfg->max_steps < 1))
     return PP_CHEM_EINVAL;
   if (cfg->workspace_bytes < 0) return PP_CHEM_EINVAL;
+  if (cfg->clean_init_massfrac != 0 && cfg->clean_init_massfrac != 1) return PP_CHEM_EINVAL;
   const int bs = cfg->block_size > 0 ? cfg->block_size : 64;
   if (bs > 64 || bs % 64 != 0) return PP_CHEM_EINVAL;  // kernels are built for 64-lane blocks
   pp_chem* h = new pp_chem();
@@ -467,32 +501,9 @@ int pp_chem_react_box(pp_chem* h, const int lo[3], const int hi[3], const pp_arr
   if (!lo || !hi || !rY || !rYsrc || !T || !rE || !rEsrc || !dt_react || !time)
     return fail(h, PP_CHEM_EINVAL, "null argument");
   if (!rY->p || !rYsrc->p || !T->p || !rE->p || !rEsrc->p) return fail(h, PP_CHEM_EINVAL, "null array");
-  long long nc = 1;
   ppc::BoxArgs b{};
-  for (int d = 0; d < 3; ++d) {
-    b.lo[d] = lo[d];
-    b.len[d] = hi[d] - lo[d] + 1;
-    if (b.len[d] < 0) return fail(h, PP_CHEM_EINVAL, "hi < lo");
-    nc *= b.len[d];
-  }
-  if (nc > 0x7fffffffLL) return fail(h, PP_CHEM_EINVAL, "box too large");
-  const int ncells = (int)nc;
-  // every view must cover the box and have enough components
-  const pp_array4* views[] = {rY, rYsrc, T, rE, rEsrc, FC};
-  const int need[] = {h->L->ns, h->L->ns, 1, 1, 1, 1};
-  for (int v = 0; v < 6; ++v) {
-    const pp_array4* a = views[v];
-    if (!a || !a->p) continue;
-    for (int d = 0; d < 3; ++d)
-      if (lo[d] < a->begin[d] || hi[d] >= a->end[d]) return fail(h, PP_CHEM_EINVAL, "Array4 does not cover the box");
-    if (a->ncomp < need[v]) return fail(h, PP_CHEM_EINVAL, "Array4 has too few components");
-  }
-  b.rY = to_a4(rY);
-  b.rYsrc = to_a4(rYsrc);
-  b.T = to_a4(T);
-  b.rE = to_a4(rE);
-  b.rEsrc = to_a4(rEsrc);
-  b.FC = to_a4(FC);
+  int ncells = 0;
+  if (int r = box_args(h, lo, hi, rY, rYsrc, T, rE, rEsrc, FC, b, ncells)) return r;
   if (int r = ensure_counters(h, ncells > 0 ? ncells : 1)) return r;
   if (int r = prepare_workspace(h, ncells)) return r;
   hipStream_t s = (hipStream_t)stream;
@@ -566,6 +577,53 @@ int pp_chem_react_1d(pp_chem* h, double* y, double* ysrc, double* rX, double* rX
   const int r = finish(h, ncells, s, ms);
   if (h->cfg.mod_reactor) *time += *dt_react;
   return r;
+}
+
+int pp_chem_flatten(pp_chem* h, const int lo[3], const int hi[3], const pp_array4* rY, const pp_array4* rYsrc,
+                    const pp_array4* T, const pp_array4* rE, const pp_array4* rEsrc, double* y_vect,
+                    double* src_vect, double* vect_energy, double* src_vect_energy, void* stream) {
+  if (!h) return PP_CHEM_EINVAL;
+  if (!lo || !hi || !rY || !rYsrc || !T || !rE || !rEsrc) return fail(h, PP_CHEM_EINVAL, "null argument");
+  if (!rY->p || !rYsrc->p || !T->p || !rE->p || !rEsrc->p) return fail(h, PP_CHEM_EINVAL, "null array");
+  ppc::FlatArgs a{};
+  int ncells = 0;
+  if (int r = box_args(h, lo, hi, rY, rYsrc, T, rE, rEsrc, nullptr, a.b, ncells)) return r;
+  if (ncells > 0 && (!y_vect || !src_vect || !vect_energy || !src_vect_energy))
+    return fail(h, PP_CHEM_EINVAL, "null flat array");
+  a.y = y_vect;
+  a.ysrc = src_vect;
+  a.e = vect_energy;
+  a.esrc = src_vect_energy;
+  a.reactor_type = h->cfg.reactor_type;
+  a.clean = h->cfg.clean_init_massfrac;
+  hipStream_t s = (hipStream_t)stream;
+  const hipError_t e = h->L->flatten(a, 0, s);
+  if (e != hipSuccess) return hip_fail(h, e, "flatten launch");
+  HIPCHK(h, hipStreamSynchronize(s));
+  return PP_CHEM_OK;
+}
+
+int pp_chem_unflatten(pp_chem* h, const int lo[3], const int hi[3], const pp_array4* rY, const pp_array4* T,
+                      const pp_array4* rE, const pp_array4* rEsrc, const pp_array4* FC, const double* y_vect,
+                      const double* vect_energy, const long long* FCunt, double dt, void* stream) {
+  if (!h) return PP_CHEM_EINVAL;
+  if (!lo || !hi || !rY || !T || !rE || !rEsrc || !FC) return fail(h, PP_CHEM_EINVAL, "null argument");
+  if (!rY->p || !T->p || !rE->p || !rEsrc->p || !FC->p) return fail(h, PP_CHEM_EINVAL, "null array");
+  ppc::FlatArgs a{};
+  int ncells = 0;
+  if (int r = box_args(h, lo, hi, rY, nullptr, T, rE, rEsrc, FC, a.b, ncells)) return r;
+  if (ncells > 0 && (!y_vect || !vect_energy || !FCunt)) return fail(h, PP_CHEM_EINVAL, "null flat array");
+  a.y = const_cast<double*>(y_vect);
+  a.e = const_cast<double*>(vect_energy);
+  a.fc = FCunt;
+  a.dt = dt;
+  a.reactor_type = h->cfg.reactor_type;
+  a.clean = h->cfg.clean_init_massfrac;
+  hipStream_t s = (hipStream_t)stream;
+  const hipError_t e = h->L->flatten(a, 1, s);
+  if (e != hipSuccess) return hip_fail(h, e, "unflatten launch");
+  HIPCHK(h, hipStreamSynchronize(s));
+  return PP_CHEM_OK;
 }
 
 int pp_chem_get_stats(const pp_chem* h, pp_chem_stats* out) {

@@ -39,7 +39,12 @@ struct MechLauncher {
   hipError_t (*adjoint_1d)(const ReactParamsDev& P, const AdjointArgs& a, hipStream_t s);
   // workspace rows per lane a mode needs (lane_ws.h); 0: none
   long long (*ws_rows)(int mode);
+  // box <-> flat state (flatten.h): unflatten = 0 flattens, 1 unflattens
+  hipError_t (*flatten)(const FlatArgs& a, int unflatten, hipStream_t s);
 };
+
+template <class M>
+hipError_t launch_flatten(const FlatArgs& a, int unflatten, hipStream_t s);
 
 template <class M>
 hipError_t launch_adjoint(const ReactParamsDev& P, const AdjointArgs& a, hipStream_t s);
@@ -106,7 +111,8 @@ hipError_t launch_box(int mode, const ReactParamsDev& P, const BoxArgs& b, doubl
                                    1,                                                          \
                                    &launch_1d<TYPE>, &launch_box<TYPE>,                        \
                                    &launch_batched_1d<TYPE>, &launch_batched_box<TYPE>,        \
-                                   &launch_adjoint<TYPE>, &integrator_ws_rows<TYPE>};          \
+                                   &launch_adjoint<TYPE>, &integrator_ws_rows<TYPE>,           \
+                                   &launch_flatten<TYPE>};                                     \
     return &L;                                                                                 \
   }                                                                                            \
   }

@@ -6,6 +6,7 @@
 #include "batched.h"
 #include "wave_bdf.h"
 #include "wave_adjoint.h"
+#include "flatten.h"
 
 namespace ppc {
 
@@ -135,7 +136,24 @@ hipError_t launch_adjoint(const ReactParamsDev& P, const AdjointArgs& a, hipStre
   }
 }
 
+template <class M>
+hipError_t launch_flatten(const FlatArgs& a, int unflatten, hipStream_t s) {
+  const long long ncells = (long long)a.b.len[0] * a.b.len[1] * a.b.len[2];
+  if (ncells <= 0) return hipSuccess;
+  const int grid = (int)((ncells + 255) / 256 < 8192 ? (ncells + 255) / 256 : 8192);
+  if (unflatten)
+    hipLaunchKernelGGL(unflatten_kernel<M>, dim3(grid), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(flatten_kernel<M>, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 }  // namespace ppc
+
+#define PPC_INSTANTIATE_FLATTEN(TYPE, MODE)                                                        \
+  namespace ppc {                                                                                 \
+  template hipError_t launch_flatten<TYPE>(const FlatArgs&, int, hipStream_t);                   \
+  }
 
 #define PPC_INSTANTIATE_ADJOINT(TYPE, MODE)                                                           \
   namespace ppc {                                                                                    \

@@ -305,7 +305,10 @@ PPC_DI double rms_norm(const double* __restrict__ x, const double* __restrict__ 
 // all issued before their stores, with a chunk of the pivot row held in
 // registers across the rows below it.  Each element receives the same
 // operations in the same order as in the textbook k-i-j loop.
-constexpr int LU_CHUNK = 8;
+#ifndef PPC_LU_CHUNK
+#define PPC_LU_CHUNK 8
+#endif
+constexpr int LU_CHUNK = PPC_LU_CHUNK;
 
 template <int n, typename T, class AM>
 PPC_HEAVY void lu_factor(const AM a, int* __restrict__ piv) {

@@ -42,6 +42,18 @@ struct AdjointArgs {
   double dt;          // per call
 };
 
+// Box <-> flat state conversion (flatten.h); device pointers.
+struct FlatArgs {
+  BoxArgs b;             // rY, rYsrc, T, rE, rEsrc, FC views over the box
+  double* y;             // ncells x (N+1) YCOrder
+  double* ysrc;          // ncells x N (flatten only)
+  double* e;             // ncells: rho e or rho h
+  double* esrc;          // ncells (flatten only)
+  const long long* fc;   // ncells step counts written to FC (unflatten only)
+  double dt;             // unflatten: rE = e + dt * rEsrc
+  int reactor_type, clean;
+};
+
 struct CellStats {
   int* nsteps;  // per cell (always)
   int* nrhs;    // optional per-cell counters (may be null)

@@ -64,6 +64,7 @@ class Config(C.Structure):
         ("rk64_nsubsteps_min", C.c_int), ("rk64_nsubsteps_max", C.c_int),
         ("rtol", C.c_double), ("atol", C.c_double), ("first_step", C.c_double),
         ("max_steps", C.c_int), ("block_size", C.c_int), ("workspace_bytes", C.c_longlong),
+        ("clean_init_massfrac", C.c_int),
     ]
 
 
@@ -94,6 +95,7 @@ _KEYS = {
     "rk64_nsubsteps_guess": "rk64_nsubsteps_guess", "rk64_nsubsteps_min": "rk64_nsubsteps_min",
     "rk64_nsubsteps_max": "rk64_nsubsteps_max", "mod_reactor": "mod_reactor",
     "block_size": "block_size", "workspace_bytes": "workspace_bytes",
+    "clean_init_massfrac": "clean_init_massfrac",
 }
 
 _lib = None
@@ -123,6 +125,10 @@ def library() -> C.CDLL:
                                          C.POINTER(C.c_double), vp]
         L.pp_chem_react_1d.argtypes = [vp, vp, vp, vp, vp, C.POINTER(C.c_double),
                                         C.POINTER(C.c_double), C.c_int, C.c_int, vp]
+        L.pp_chem_flatten.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), a4, a4, a4, a4, a4,
+                                      vp, vp, vp, vp, vp]
+        L.pp_chem_unflatten.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), a4, a4, a4, a4, a4,
+                                        vp, vp, vp, C.c_double, vp]
         L.pp_chem_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.pp_chem_copy_cell_steps.argtypes = [vp, vp, C.c_int]
         L.pp_chem_last_error.argtypes = [vp]
@@ -202,6 +208,24 @@ class ReactorBase:
         self.nspec = L.pp_chem_num_species(self._h)
         return 0
 
+    def set_typ_vals_ode(self, typ_vals):
+        """ReactorBase::set_typ_vals_ode (ReactorBase.cpp:6-31): N+1 typical
+        values (species, then T).  Like the reference's ReactorBDF and
+        ReactorRK64, no integrator here reads them (only CVODE / ARKODE
+        tolerances do); they are kept for callers that set them."""
+        vals = [float(v) for v in typ_vals]
+        if len(vals) != self.nspec + 1:
+            raise ReactorError(-1, f"set_typ_vals_ode: expected {self.nspec + 1} values, got {len(vals)}")
+        self.typ_vals = vals
+
+    def set_eos_parm(self, eosparm):
+        """ReactorBase::set_eos_parm (ReactorBase.H:86-97): a parameter block
+        whose identifier is "Manifold" is rejected, as the reference aborts for
+        these reactors; any other (the ideal-gas EOS takes none) is kept."""
+        if getattr(eosparm, "identifier", None) == "Manifold":
+            raise ReactorError(-1, "Manifold EOS only supported with ReactorRK64 and ReactorNull for now")
+        self.eosparm = eosparm
+
     def close(self):
         if self._h:
             library().pp_chem_destroy(self._h)
@@ -231,6 +255,40 @@ class ReactorBase:
         rc = library().pp_chem_react_box(self._h, lo, hi, *[C.byref(v) for v in views], C.byref(fc),
                                          None, C.byref(dt), C.byref(tm), stream)
         return self._check(rc), tm.value
+
+    def flatten(self, rY, rYsrc, T, rE, rEsrc, stream: Optional[int] = None):
+        """Box -> flat (YCOrder) state of device tensors shaped (ncomp, nz, ny, nx)
+        (ReactorBase::flatten); returns torch tensors (y_vect, src_vect,
+        vect_energy, src_vect_energy) on the same device."""
+        import torch
+        nz, ny, nx = T.shape[-3:]
+        ncells, ns = nx * ny * nz, self.nspec
+        lo = (C.c_int * 3)(0, 0, 0)
+        hi = (C.c_int * 3)(nx - 1, ny - 1, nz - 1)
+        views = [_array4_from_tensor(t) for t in (rY, rYsrc, T, rE, rEsrc)]
+        opts = dict(dtype=torch.float64, device=T.device)
+        y, ys = torch.empty(ncells * (ns + 1), **opts), torch.empty(ncells * ns, **opts)
+        e, es = torch.empty(ncells, **opts), torch.empty(ncells, **opts)
+        rc = library().pp_chem_flatten(self._h, lo, hi, *[C.byref(v) for v in views], y.data_ptr(),
+                                       ys.data_ptr(), e.data_ptr(), es.data_ptr(), stream)
+        self._check(rc)
+        return y, ys, e, es
+
+    def unflatten(self, rY, T, rE, rEsrc, FC, y_vect, vect_energy, FCunt, dt_react: float,
+                  stream: Optional[int] = None):
+        """Flat -> box state (ReactorBase::unflatten): rY, T, rE, FC updated in
+        place; FCunt is an int64 device tensor of per-cell step counts."""
+        import torch
+        nz, ny, nx = T.shape[-3:]
+        lo = (C.c_int * 3)(0, 0, 0)
+        hi = (C.c_int * 3)(nx - 1, ny - 1, nz - 1)
+        views = [_array4_from_tensor(t) for t in (rY, T, rE, rEsrc, FC)]
+        if FCunt.dtype != torch.int64:
+            raise ValueError("FCunt must be int64")
+        rc = library().pp_chem_unflatten(self._h, lo, hi, *[C.byref(v) for v in views], y_vect.data_ptr(),
+                                         vect_energy.data_ptr(), FCunt.data_ptr(), C.c_double(dt_react),
+                                         stream)
+        self._check(rc)
 
     def react_1d(self, y, ysrc, rX, rXsrc, dt_react: float, time: float = 0.0,
                  stream: Optional[int] = None):

@@ -1,0 +1,97 @@
+"""Box <-> flat conversion on the GPU (pp_chem_flatten / pp_chem_unflatten)
+against the oracle's restatement of utils::box_flatten / box_unflatten
+(Source/Reactions/ReactorUtils.H:160-289), with and without
+clean_init_massfrac, for both reactor types.  Species, sources, energies and
+step counts are copied or combined by the same few operations and must agree
+to the last bits (1e-14 relative); the temperatures come from the same
+Newton inversion, which stops at |dT| < 1e-6 K, and must agree to 2e-6 K.
+"""
+import numpy as np
+import pytest
+
+from pelephysics_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _box(st, rY, nx, ny, nz):
+    import torch
+    dev = torch.device("cuda:0")
+
+    def t(a2d):
+        return torch.from_numpy(np.ascontiguousarray(a2d.T.reshape(-1, nz, ny, nx))).to(dev)
+
+    return t
+
+
+@pytest.mark.parametrize("reactor_type", [1, 2])
+@pytest.mark.parametrize("clean", [0, 1])
+def test_flatten_unflatten_parity(models, oracles, reactor_type, clean):
+    import torch
+    assert torch.cuda.is_available()
+    from pelephysics_amd.reactor import ReactorBase
+    m, O = models["drm19"], oracles["drm19"]
+    ns = m.n_species
+    nx, ny, nz = 5, 3, 4
+    n = nx * ny * nz
+    st = W.drm19_fresh(m, n, seed=12)
+    rng = np.random.default_rng(12)
+    rY = st.rY.copy()
+    rY[:, 7] = -1e-9 * rng.uniform(0.1, 1.0, n)
+    T = st.T + rng.uniform(-30.0, 30.0, n)
+    rE = st.rE.copy() if reactor_type == 2 else st.rE - st.p
+    rYsrc = rng.standard_normal(rY.shape) * 1e-3
+    rEsrc = rng.standard_normal(n) * 1e5
+    t = _box(st, rY, nx, ny, nz)
+    bY, bYs, bT, bE, bEs = t(rY), t(rYsrc), t(T[:, None]), t(rE[:, None]), t(rEsrc[:, None])
+    bFC = torch.zeros_like(bT)
+    R = ReactorBase.create("ReactorBDF")
+    R.init(reactor_type, n, mechanism="drm19", clean_init_massfrac=clean)
+
+    y, ys, e, es = R.flatten(bY, bYs, bT, bE, bEs)
+    oy, oys, oe, oes = O.flatten(rY, rYsrc, T, rE, rEsrc, reactor_type, bool(clean))
+    gy = y.cpu().numpy().reshape(n, ns + 1)
+    oy2 = oy.reshape(n, ns + 1)
+    assert np.array_equal(gy[:, :ns], oy2[:, :ns])
+    assert np.array_equal(ys.cpu().numpy(), oys) and np.array_equal(e.cpu().numpy(), oe)
+    assert np.array_equal(es.cpu().numpy(), oes)
+    np.testing.assert_allclose(gy[:, ns], oy2[:, ns], rtol=0, atol=2e-6)
+
+    # an integration result: mass-conserving increments and step counts
+    d = rng.standard_normal((n, ns)) * 1e-7
+    d -= d.mean(1, keepdims=True)
+    y2 = oy2.copy()
+    y2[:, :ns] += d
+    y2[:, ns] += 5.0
+    fc = rng.integers(1, 400, n).astype(np.int64)
+    dt = 1e-5
+    R.unflatten(bY, bT, bE, bEs, bFC, torch.from_numpy(y2.reshape(-1)).cuda(), e,
+                torch.from_numpy(fc).cuda(), dt)
+    orY, oT, orE = rY.copy(), T.copy(), rE.copy()
+    oFC = O.unflatten(orY, oT, orE, rEsrc, y2.reshape(-1), oe, fc, dt, reactor_type, bool(clean))
+    gY = bY.cpu().numpy().reshape(ns, -1).T
+    np.testing.assert_allclose(gY, orY, rtol=1e-14, atol=1e-25)
+    np.testing.assert_allclose(bE.cpu().numpy().reshape(-1), orE, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(bT.cpu().numpy().reshape(-1), oT, rtol=0, atol=2e-6)
+    assert np.array_equal(bFC.cpu().numpy().reshape(-1), oFC)
+
+
+def test_typ_vals_and_eos_parm():
+    """ReactorBase::set_typ_vals_ode takes N+1 values (ReactorBase.cpp:9) and
+    set_eos_parm rejects the Manifold EOS (ReactorBase.H:90-94)."""
+    import torch
+    assert torch.cuda.is_available()
+    from pelephysics_amd.reactor import ReactorBase, ReactorError
+    R = ReactorBase.create("ReactorRK64")
+    R.init(2, 4, mechanism="LiDryer")
+    R.set_typ_vals_ode([1e-3] * 9 + [1500.0])
+    assert R.typ_vals[-1] == 1500.0
+    with pytest.raises(ReactorError):
+        R.set_typ_vals_ode([1e-3] * 9)
+
+    class Manifold:
+        identifier = "Manifold"
+
+    with pytest.raises(ReactorError):
+        R.set_eos_parm(Manifold())
+    R.set_eos_parm(None)
