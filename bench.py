@@ -37,14 +37,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(ident, st, m, steps, warmup, rtol, atol):
+def run_gpu(ident, st, m, steps, warmup, rtol, atol, mech=MECH):
     import torch
     from pelephysics_amd.reactor import ReactorBase
     dev = torch.device("cuda:0")
     ns = m.n_species
     n = GRID
     R = ReactorBase.create(ident)
-    R.init(2, st.ncells, mechanism=MECH, rtol=rtol, atol=atol)
+    R.init(2, st.ncells, mechanism=mech, rtol=rtol, atol=atol)
 
     def soa(a, nc):
         return torch.from_numpy(np.ascontiguousarray(a.T.reshape(nc, n, n, n))).to(dev)
@@ -201,6 +201,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", choices=["mixed", "batched-fp64", "fp64"], default="mixed")
+    ap.add_argument("--impl", choices=["unrolled", "table"], default="unrolled",
+                    help="drm19 kernels of the measured form: generated straight-line code or constant tables")
+    ap.add_argument("--plain-impl", choices=["unrolled", "table"], default="unrolled",
+                    help="drm19 kernels of the plain FP64 form")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -223,12 +227,13 @@ def main():
     ident = {"mixed": "ReactorBDFAdaptiveMixed", "batched-fp64": "ReactorBDFBatchedFP64",
              "fp64": "ReactorBDFAdaptive"}[a.mode]
     log(f"[bench] {ident} drm19 {GRID}^3 = {ncells} cells, warmup {a.warmup}, steps {a.steps}")
-    head = run_gpu(ident, st, m, a.steps, a.warmup, RTOL, ATOL)
+    impl = {"unrolled": MECH, "table": MECH + "_table"}
+    head = run_gpu(ident, st, m, a.steps, a.warmup, RTOL, ATOL, impl[a.impl])
     value = ncells * a.steps / head["elapsed"]
     log(f"[bench] {ident}: {value:.4g} cells/s, kernel {head['kernel_ms']:.2f} ms")
     extra = {}
     if a.mode != "fp64" and not a.no_plain:
-        plain = run_gpu("ReactorBDFAdaptive", st, m, max(1, a.steps // 2), 1, RTOL, ATOL)
+        plain = run_gpu("ReactorBDFAdaptive", st, m, max(1, a.steps // 2), 1, RTOL, ATOL, impl[a.plain_impl])
         pv = ncells * max(1, a.steps // 2) / plain["elapsed"]
         Yp = plain["rY"] / plain["rY"].sum(1, keepdims=True)
         Ym = head["rY"] / head["rY"].sum(1, keepdims=True)
@@ -256,7 +261,8 @@ def main():
         "config": {"workload": f"drm19 {GRID}^3 Gaussian-random-field mixture fraction x progress "
                                f"variable, h-reactor, dt=1e-5 s, adaptive BDF rtol={RTOL:g} "
                                f"atol={ATOL:g}",
-                   "integrator": ident, "cells": ncells, "parallelism": "single GPU"},
+                   "integrator": ident, "kernels": impl[a.impl], "plain_kernels": impl[a.plain_impl],
+                   "cells": ncells, "parallelism": "single GPU"},
         "per_cell": {k: s[k] / ncells for k in ("nsteps", "nrhs", "njac", "nlu", "nnewton", "nreject")},
         "kernel_ms": head["kernel_ms"],
         "roofline": rl, "cpu_baseline": cpu,

@@ -30,11 +30,14 @@ GENERATORS = {"LiDryer": W.lidryer_h2air, "dodecane_lu": W.dodecane_uneven,
 
 
 def workload(mech: str, ncells: int) -> W.CellState:
-    m = load(mech)
-    if mech == "drm19":
+    """State for a compiled mechanism name (a "_table" suffix selects the
+    table-driven form of the same mechanism)."""
+    base = mech[:-len("_table")] if mech.endswith("_table") else mech
+    m = load(base)
+    if base == "drm19":
         side = max(1, round(ncells ** (1.0 / 3.0)))
         return W.drm19_grf(m, side)
-    return GENERATORS[mech](m, ncells)
+    return GENERATORS[base](m, ncells)
 
 
 def main(argv=None) -> dict:
@@ -51,7 +54,7 @@ def main(argv=None) -> dict:
 
     import torch
     st = workload(a.mech, a.cells)
-    m = load(a.mech)
+    m = load(a.mech[:-len("_table")] if a.mech.endswith("_table") else a.mech)
     dev = torch.device("cuda:0")
     R = ReactorBase.create(a.integrator)
     R.init(2, st.ncells, mechanism=a.mech, rtol=a.rtol, atol=a.atol)
