@@ -134,6 +134,17 @@ int pp_chem_unflatten(pp_chem* h, const int lo[3], const int hi[3], const pp_arr
                       const pp_array4* rE, const pp_array4* rEsrc, const pp_array4* FC, const double* y_vect,
                       const double* vect_energy, const long long* FCunt, double dt, void* stream);
 
+/* ncalls successive 1-D calls of dt_react on device arrays (a flow solver's
+ * chemistry steps without host work in between): the same results as ncalls
+ * calls of pp_chem_react_1d (rX advanced by dt * rXsrc each call).  One call's
+ * device work (queue reset, cost ordering, integration kernel) is captured
+ * once as a HIP graph and replayed; the graph is rebuilt when an array,
+ * ncells or dt changes.  Statistics and per-cell counters are those of the
+ * last call; kernel_ms covers all calls.  No counterpart in the reference,
+ * whose callers loop over ReactorBDF::react (ReactEval/main.cpp:475-686). */
+int pp_chem_react_1d_chained(pp_chem* h, double* y, double* ysrc, double* rX, double* rXsrc,
+                             double* dt_react, double* time, int ncells, int ncalls, void* stream);
+
 int pp_chem_get_stats(const pp_chem* h, pp_chem_stats* out);
 int pp_chem_num_reactions(const pp_chem* h);
 

@@ -130,6 +130,19 @@ __global__ void __launch_bounds__(256) cost_scatter(const int* __restrict__ nrhs
 
 }  // namespace
 
+// What a captured chained-call graph depends on.
+struct GraphKey {
+  const void* arrays[4];
+  int ncells;
+  double dt;
+  const void *cnt, *order, *ws;
+  long long ws_lanes;
+  bool operator==(const GraphKey& o) const {
+    return std::memcmp(arrays, o.arrays, sizeof(arrays)) == 0 && ncells == o.ncells && dt == o.dt &&
+           cnt == o.cnt && order == o.order && ws == o.ws && ws_lanes == o.ws_lanes;
+  }
+};
+
 struct pp_chem {
   pp_chem_config cfg;
   std::string mech_name;
@@ -159,6 +172,13 @@ struct pp_chem {
   // per-lane matrix workspace of the larger mechanisms, grown on demand
   size_t ws_cap = 0;
   double* d_ws = nullptr;
+  // chained 1-D calls: one call's device work captured as a graph on a
+  // stream of the handle, replayed; rebuilt when any captured pointer changes
+  hipStream_t gstream = nullptr;
+  hipEvent_t gev0 = nullptr, gev1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  GraphKey gkey{};
 };
 
 namespace {
@@ -285,6 +305,24 @@ ppc::QueueArgs prepare_queue(pp_chem* h, int ncells, const void* key, hipStream_
   h->hist_ncells = ncells;
   h->hist_key = key;
   return q;
+}
+
+// One 1-D call's device work on stream s (the queue in cost order when the
+// previous call integrated the same cells).
+hipError_t enqueue_1d(pp_chem* h, double* y, double* ysrc, double* rX, double* rXsrc, double dt, int ncells,
+                      const void* key, hipStream_t s) {
+  if (is_batched(h->cfg.mode)) {
+    const ppc::QueueArgs q = prepare_queue(h, ncells, key, s);
+    return h->L->batched_1d(h->cfg.mode, h->P, y, ysrc, rX, rXsrc, dt, q, counters(h), s);
+  }
+  return h->L->react_1d(h->cfg.mode, h->P, y, ysrc, rX, rXsrc, dt, ncells, counters(h), h->cfg.block_size, s);
+}
+
+void drop_graph(pp_chem* h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
 }
 
 ppc::ReactParamsDev to_dev(const pp_chem_config& c) {
@@ -437,6 +475,10 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
 
 void pp_chem_destroy(pp_chem* h) {
   if (!h) return;
+  drop_graph(h);
+  if (h->gstream) (void)hipStreamDestroy(h->gstream);
+  if (h->gev0) (void)hipEventDestroy(h->gev0);
+  if (h->gev1) (void)hipEventDestroy(h->gev1);
   if (h->d_ckpt) (void)hipFree(h->d_ckpt);
   if (h->d_ws) (void)hipFree(h->d_ws);
   if (h->d_cnt) (void)hipFree(h->d_cnt);
@@ -557,13 +599,7 @@ int pp_chem_react_1d(pp_chem* h, double* y, double* ysrc, double* rX, double* rX
   }
   HIPCHK(h, hipEventRecord(h->ev0, s));
   if (ncells > 0) {
-    hipError_t e;
-    if (is_batched(h->cfg.mode)) {
-      const ppc::QueueArgs q = prepare_queue(h, ncells, y, s);
-      e = h->L->batched_1d(h->cfg.mode, h->P, dy, dys, dx, dxs, *dt_react, q, counters(h), s);
-    } else {
-      e = h->L->react_1d(h->cfg.mode, h->P, dy, dys, dx, dxs, *dt_react, ncells, counters(h), h->cfg.block_size, s);
-    }
+    const hipError_t e = enqueue_1d(h, dy, dys, dx, dxs, *dt_react, ncells, y, s);
     if (e != hipSuccess) return hip_fail(h, e, "react_1d launch");
   }
   HIPCHK(h, hipEventRecord(h->ev1, s));
@@ -624,6 +660,61 @@ int pp_chem_unflatten(pp_chem* h, const int lo[3], const int hi[3], const pp_arr
   if (e != hipSuccess) return hip_fail(h, e, "unflatten launch");
   HIPCHK(h, hipStreamSynchronize(s));
   return PP_CHEM_OK;
+}
+
+int pp_chem_react_1d_chained(pp_chem* h, double* y, double* ysrc, double* rX, double* rXsrc, double* dt_react,
+                             double* time, int ncells, int ncalls, void* stream) {
+  if (!h) return PP_CHEM_EINVAL;
+  if (!dt_react || !time || ncells < 0 || ncalls < 1) return fail(h, PP_CHEM_EINVAL, "bad argument");
+  if (ncells > 0 && (!y || !ysrc || !rX || !rXsrc)) return fail(h, PP_CHEM_EINVAL, "null array");
+  hipStream_t s = (hipStream_t)stream;
+  if (int r = ensure_counters(h, ncells > 0 ? ncells : 1)) return r;
+  if (int r = prepare_workspace(h, ncells)) return r;
+  if (!h->gstream) {
+    HIPCHK(h, hipStreamCreateWithFlags(&h->gstream, hipStreamNonBlocking));
+    HIPCHK(h, hipEventCreateWithFlags(&h->gev0, hipEventDisableTiming));
+    HIPCHK(h, hipEventCreateWithFlags(&h->gev1, hipEventDisableTiming));
+  }
+  const double dt = *dt_react;
+  HIPCHK(h, hipEventRecord(h->ev0, s));
+  int done = 0;
+  if (ncells > 0 && is_batched(h->cfg.mode) &&
+      !(h->hist_valid && h->hist_ncells == ncells && h->hist_key == y)) {
+    // the first call sets the cost history the captured calls order by
+    const hipError_t e = enqueue_1d(h, y, ysrc, rX, rXsrc, dt, ncells, y, s);
+    if (e != hipSuccess) return hip_fail(h, e, "react_1d launch");
+    done = 1;
+  }
+  if (ncells > 0 && done < ncalls) {
+    const GraphKey key{{y, ysrc, rX, rXsrc}, ncells, dt, h->d_cnt, h->d_order, h->P.ws, h->P.ws_lanes};
+    if (!h->gexec || !(key == h->gkey)) {
+      drop_graph(h);
+      HIPCHK(h, hipStreamBeginCapture(h->gstream, hipStreamCaptureModeThreadLocal));
+      const hipError_t e = enqueue_1d(h, y, ysrc, rX, rXsrc, dt, ncells, y, h->gstream);
+      hipGraph_t g = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(h->gstream, &g);
+      if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(h, e, "react_1d capture");
+      }
+      if (e2 != hipSuccess) return hip_fail(h, e2, "hipStreamEndCapture");
+      h->graph = g;
+      HIPCHK(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+      h->gkey = key;
+    }
+    HIPCHK(h, hipEventRecord(h->gev0, s));
+    HIPCHK(h, hipStreamWaitEvent(h->gstream, h->gev0, 0));
+    for (int c = done; c < ncalls; ++c) HIPCHK(h, hipGraphLaunch(h->gexec, h->gstream));
+    HIPCHK(h, hipEventRecord(h->gev1, h->gstream));
+    HIPCHK(h, hipStreamWaitEvent(s, h->gev1, 0));
+  }
+  HIPCHK(h, hipEventRecord(h->ev1, s));
+  HIPCHK(h, hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  const int r = finish(h, ncells, s, ms);
+  if (h->cfg.mod_reactor) *time += ncalls * dt;
+  return r;
 }
 
 int pp_chem_get_stats(const pp_chem* h, pp_chem_stats* out) {

@@ -129,6 +129,8 @@ def library() -> C.CDLL:
                                       vp, vp, vp, vp, vp]
         L.pp_chem_unflatten.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), a4, a4, a4, a4, a4,
                                         vp, vp, vp, C.c_double, vp]
+        L.pp_chem_react_1d_chained.argtypes = [vp, vp, vp, vp, vp, C.POINTER(C.c_double),
+                                                C.POINTER(C.c_double), C.c_int, C.c_int, vp]
         L.pp_chem_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.pp_chem_copy_cell_steps.argtypes = [vp, vp, C.c_int]
         L.pp_chem_last_error.argtypes = [vp]
@@ -310,6 +312,20 @@ class ReactorBase:
         tm = C.c_double(time)
         rc = library().pp_chem_react_1d(self._h, *ptrs, C.byref(dt), C.byref(tm), ncells, on_dev,
                                         stream)
+        return self._check(rc), tm.value
+
+    def react_1d_chained(self, y, ysrc, rX, rXsrc, dt_react: float, ncalls: int, time: float = 0.0,
+                         stream: Optional[int] = None):
+        """ncalls successive 1-D calls on device tensors, replayed from a HIP
+        graph of one call; returns (avg steps of the last call, time)."""
+        for a in (y, ysrc, rX, rXsrc):
+            if isinstance(a, np.ndarray) or not a.is_contiguous():
+                raise ValueError("chained calls take contiguous device tensors")
+        dt = C.c_double(dt_react)
+        tm = C.c_double(time)
+        rc = library().pp_chem_react_1d_chained(self._h, y.data_ptr(), ysrc.data_ptr(), rX.data_ptr(),
+                                                rXsrc.data_ptr(), C.byref(dt), C.byref(tm),
+                                                int(rX.shape[0]), int(ncalls), stream)
         return self._check(rc), tm.value
 
     def adjoint_1d(self, y, rX, nmacro: int, dt_react: float, species=("CO", "OH"),

@@ -1,7 +1,7 @@
 """Throughput of one integrator on a synthetic workload, state resident on the device.
 
     python -m pelephysics_amd.throughput --mech heptane_lu_88sk --cells 131072 \\
-        --integrator ReactorBDFAdaptiveMixed --calls 3 [--chained] [--rtol 1e-10 --atol 1e-16]
+        --integrator ReactorBDFAdaptiveMixed --calls 3 [--chained [--graph]] [--rtol 1e-10 --atol 1e-16]
 
 Workloads (pelephysics_amd.workloads): LiDryer -> lidryer_h2air, drm19 ->
 drm19_grf (cells rounded to a cube), dodecane_lu -> dodecane_uneven,
@@ -47,6 +47,8 @@ def main(argv=None) -> dict:
     ap.add_argument("--integrator", default="ReactorBDFAdaptiveMixed")
     ap.add_argument("--calls", type=int, default=3)
     ap.add_argument("--chained", action="store_true")
+    ap.add_argument("--graph", action="store_true",
+                    help="with --chained: all calls in one pp_chem_react_1d_chained (HIP graph replay)")
     ap.add_argument("--dt", type=float, default=1e-5)
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--atol", type=float, default=1e-16)
@@ -65,6 +67,18 @@ def main(argv=None) -> dict:
     xs = torch.zeros(st.ncells, dtype=torch.float64, device=dev)
     kms, per_call = [], []
     t0 = time.perf_counter()
+    if a.chained and a.graph:
+        R.react_1d_chained(y, ys, x, xs, a.dt, a.calls)
+        s = R.stats()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        out = {"mech": a.mech, "integrator": a.integrator, "cells": st.ncells, "dt": a.dt, "calls": a.calls,
+               "chained": True, "graph": True, "kernel_ms_total": s["kernel_ms"], "wall_s": wall,
+               "cells_per_s": st.ncells * a.calls / wall,
+               "finite": bool(torch.isfinite(y).all().item())}
+        R.close()
+        print(json.dumps(out))
+        return out
     for call in range(a.calls):
         if not a.chained:
             y.copy_(y0)
@@ -82,6 +96,7 @@ def main(argv=None) -> dict:
         "mech": a.mech, "integrator": a.integrator, "cells": st.ncells, "dt": a.dt,
         "rtol": a.rtol, "atol": a.atol, "calls": a.calls, "chained": a.chained,
         "cells_per_s": st.ncells / (float(np.median(kms)) * 1e-3),
+        "cells_per_s_wall": st.ncells * a.calls / wall,
         "kernel_ms": kms, "wall_s": wall, "per_cell": per_call,
         "workspace_bytes_per_resident_cell": R.workspace_rows() * 8,
         "finite": bool(torch.isfinite(y).all().item()),
