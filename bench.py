@@ -37,14 +37,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(ident, st, m, steps, warmup, rtol, atol, mech=MECH):
+def run_gpu(ident, st, m, steps, warmup, rtol, atol, mech=MECH, waves=0):
     import torch
     from pelephysics_amd.reactor import ReactorBase
     dev = torch.device("cuda:0")
     ns = m.n_species
     n = GRID
     R = ReactorBase.create(ident)
-    R.init(2, st.ncells, mechanism=mech, rtol=rtol, atol=atol)
+    R.init(2, st.ncells, mechanism=mech, rtol=rtol, atol=atol, batched_waves_per_cu=waves)
 
     def soa(a, nc):
         return torch.from_numpy(np.ascontiguousarray(a.T.reshape(nc, n, n, n))).to(dev)
@@ -85,16 +85,19 @@ def run_gpu(ident, st, m, steps, warmup, rtol, atol, mech=MECH):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_drm19_64cubed.json")
 
 
-def measured_traffic(ident):
+def measured_traffic(ident, kernels, waves):
     """HBM bytes per launch of the integration kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/), or None."""
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/), or None when they
+    were taken on other kernels or another launch shape."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
     e = t.get(ident)
-    return None if e is None else float(e["bytes_per_launch"])
+    if e is None or e.get("kernels") != kernels or e.get("batched_waves_per_cu", 0) != waves:
+        return None
+    return float(e["bytes_per_launch"])
 
 
 def copy_only(nbytes):
@@ -201,9 +204,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", choices=["mixed", "batched-fp64", "fp64"], default="mixed")
-    ap.add_argument("--impl", choices=["unrolled", "table"], default="unrolled",
-                    help="drm19 kernels of the measured form: generated straight-line code or constant tables")
-    ap.add_argument("--plain-impl", choices=["unrolled", "table"], default="unrolled",
+    ap.add_argument("--impl", choices=["tuned", "unrolled", "table"], default="tuned",
+                    help="drm19 kernels of the measured form: the autotuner's choice (pelephysics_amd/tuning), "
+                         "generated straight-line code or constant tables")
+    ap.add_argument("--plain-impl", choices=["tuned", "unrolled", "table"], default="tuned",
                     help="drm19 kernels of the plain FP64 form")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -227,13 +231,24 @@ def main():
     ident = {"mixed": "ReactorBDFAdaptiveMixed", "batched-fp64": "ReactorBDFBatchedFP64",
              "fp64": "ReactorBDFAdaptive"}[a.mode]
     log(f"[bench] {ident} drm19 {GRID}^3 = {ncells} cells, warmup {a.warmup}, steps {a.steps}")
-    impl = {"unrolled": MECH, "table": MECH + "_table"}
-    head = run_gpu(ident, st, m, a.steps, a.warmup, RTOL, ATOL, impl[a.impl])
+    def choose(kind, integrator):
+        """(kernels, waves per CU, tuning entry or None) of one measured form."""
+        if kind == "tuned":
+            from pelephysics_amd.autotune import lookup
+            e = lookup(MECH, integrator, "box")
+            if e:
+                return e["kernels"], e["batched_waves_per_cu"], {k: e[k] for k in e if k != "candidates"}
+            return MECH, 0, None
+        return {"unrolled": MECH, "table": MECH + "_table"}[kind], 0, None
+
+    kern, waves, tuning = choose(a.impl, ident)
+    pkern, pwaves, ptuning = choose(a.plain_impl, "ReactorBDFAdaptive")
+    head = run_gpu(ident, st, m, a.steps, a.warmup, RTOL, ATOL, kern, waves)
     value = ncells * a.steps / head["elapsed"]
     log(f"[bench] {ident}: {value:.4g} cells/s, kernel {head['kernel_ms']:.2f} ms")
     extra = {}
     if a.mode != "fp64" and not a.no_plain:
-        plain = run_gpu("ReactorBDFAdaptive", st, m, max(1, a.steps // 2), 1, RTOL, ATOL, impl[a.plain_impl])
+        plain = run_gpu("ReactorBDFAdaptive", st, m, max(1, a.steps // 2), 1, RTOL, ATOL, pkern, pwaves)
         pv = ncells * max(1, a.steps // 2) / plain["elapsed"]
         Yp = plain["rY"] / plain["rY"].sum(1, keepdims=True)
         Ym = head["rY"] / head["rY"].sum(1, keepdims=True)
@@ -244,7 +259,7 @@ def main():
         log(f"[bench] plain FP64: {pv:.4g} cells/s; speedup {value / pv:.3f}")
     s = head["stats"]
     rl = roofline(m, s, head["kernel_ms"], mixed)
-    rl["traffic"] = measured_traffic(ident)
+    rl["traffic"] = measured_traffic(ident, kern, waves)
     rl["algorithmic_bytes"] = (3 * m.n_species + 6) * 8 * ncells  # (3N+6)*8 B per cell: read+write once
     adj = None if a.no_adjoint else adjoint_measure(m)
     if adj:
@@ -261,7 +276,8 @@ def main():
         "config": {"workload": f"drm19 {GRID}^3 Gaussian-random-field mixture fraction x progress "
                                f"variable, h-reactor, dt=1e-5 s, adaptive BDF rtol={RTOL:g} "
                                f"atol={ATOL:g}",
-                   "integrator": ident, "kernels": impl[a.impl], "plain_kernels": impl[a.plain_impl],
+                   "integrator": ident, "kernels": kern, "batched_waves_per_cu": waves, "tuning": tuning,
+                   "plain_kernels": pkern, "plain_tuning": ptuning,
                    "cells": ncells, "parallelism": "single GPU"},
         "per_cell": {k: s[k] / ncells for k in ("nsteps", "nrhs", "njac", "nlu", "nnewton", "nreject")},
         "kernel_ms": head["kernel_ms"],

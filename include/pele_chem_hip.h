@@ -74,6 +74,10 @@ typedef struct {
    * box <-> flat conversion clips rhoY to [0, rho] and re-applies the
    * increment conservatively (pp_chem_flatten / pp_chem_unflatten). */
   int clean_init_massfrac;
+  /* Batched forms: persistent waves per compute unit (0: every wave that fits,
+   * 8 at the kernels' 2 waves per SIMD); a tuning parameter
+   * (python -m pelephysics_amd.autotune). */
+  int batched_waves_per_cu;
 } pp_chem_config;
 
 /* amrex::Array4<Real> layout: element (i,j,k,n) is at

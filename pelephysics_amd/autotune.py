@@ -1,0 +1,157 @@
+"""Autotuner for the kernel implementation and launch shape of an integrator.
+
+    python -m pelephysics_amd.autotune --mech drm19 --integrator ReactorBDFAdaptiveMixed \\
+        --form box --cells 262144 [--out tuning.json]
+
+For one (mechanism, integrator, form) it times every candidate on a synthetic
+state of the mechanism's workload (pelephysics_amd.throughput.workload) and
+keeps the fastest:
+
+* kernels: the compiled implementations of the mechanism, the generated
+  straight-line code ("<mech>") and the constant-memory tables
+  ("<mech>_table"), where both exist;
+* batched_waves_per_cu (batched forms only): persistent waves per compute
+  unit, 8 (every wave that fits at 2 waves per SIMD), 6 or 4.  Fewer waves
+  contend less for the caches that back private memory but hide less latency.
+
+Each candidate runs `calls` calls on the same cells and arrays; its time is
+the kernel time of the last one (the queue in the cost order of the call
+before).  Results go to a JSON table
+keyed by the GPU architecture (gcnArchName) and "<mech>|<integrator>|<form>";
+``ReactorBase.init(..., tuned="box" or "1d")`` and bench.py read it back
+(pelephysics_amd/tuning/<arch>.json unless PELE_CHEM_TUNING names another
+file).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BATCHED = ("ReactorBDFAdaptiveMixed", "ReactorBDFBatchedFP64")
+WAVES = (8, 6, 4)
+
+
+def arch() -> str:
+    import torch
+    return torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
+
+
+def table_path(a: str | None = None) -> str:
+    return os.environ.get("PELE_CHEM_TUNING") or os.path.join(HERE, "tuning", f"{a or 'gfx950'}.json")
+
+
+def key(mech: str, integrator: str, form: str) -> str:
+    return f"{mech}|{integrator}|{form}"
+
+
+def lookup(mech: str, integrator: str, form: str = "box", a: str | None = None) -> dict | None:
+    """The tuned entry for (mechanism, integrator, form), or None."""
+    p = table_path(a)
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get(key(mech, integrator, form))
+
+
+def candidates(mech: str, integrator: str) -> list:
+    from .reactor import mechanisms
+    have = set(mechanisms())
+    impls = [k for k in (mech, mech + "_table") if k in have]
+    waves = WAVES if integrator in BATCHED else (0,)
+    return [{"kernels": k, "batched_waves_per_cu": w} for k in impls for w in waves]
+
+
+def _time(cand: dict, integrator: str, form: str, st, m, dt: float, rtol: float, atol: float,
+          calls: int) -> float:
+    import torch
+    from .reactor import ReactorBase
+    dev = torch.device("cuda:0")
+    R = ReactorBase.create(integrator)
+    R.init(2, st.ncells, mechanism=cand["kernels"], rtol=rtol, atol=atol,
+           batched_waves_per_cu=cand["batched_waves_per_cu"])
+    ns = m.n_species
+    ms = 0.0
+    if form == "box":
+        n = st.ncells
+        shape = (1, 1, n)
+
+        def soa(a, nc):
+            return torch.from_numpy(np.ascontiguousarray(a.T.reshape((nc,) + shape))).to(dev)
+
+        rY0, T0, rE0 = soa(st.rY, ns), soa(st.T[:, None], 1), soa(st.rE[:, None], 1)
+        rYs, rEs, FC = torch.zeros_like(rY0), torch.zeros_like(rE0), torch.zeros_like(T0)
+        rY, T, rE = rY0.clone(), T0.clone(), rE0.clone()  # the same arrays each call: cost order
+        for _ in range(calls):
+            rY.copy_(rY0)
+            T.copy_(T0)
+            rE.copy_(rE0)
+            R.react(rY, rYs, T, rE, rEs, FC, dt)
+            ms = R.stats()["kernel_ms"]
+    else:
+        y0 = torch.from_numpy(st.y_1d()).to(dev)
+        x0 = torch.from_numpy(st.rE.copy()).to(dev)
+        ys = torch.zeros(st.ncells * ns, dtype=torch.float64, device=dev)
+        xs = torch.zeros(st.ncells, dtype=torch.float64, device=dev)
+        y, x = y0.clone(), x0.clone()
+        for _ in range(calls):
+            y.copy_(y0)
+            x.copy_(x0)
+            R.react_1d(y, ys, x, xs, dt)
+            ms = R.stats()["kernel_ms"]
+    R.close()
+    return ms
+
+
+def tune(mech: str, integrator: str, form: str = "box", ncells: int = 262144, dt: float = 1e-5,
+         rtol: float = 1e-10, atol: float = 1e-16, calls: int = 2, out: str | None = None,
+         log=print) -> dict:
+    from .mechgen.__main__ import load
+    from .throughput import workload
+    st = workload(mech, ncells)
+    m = load(mech)
+    results = []
+    for cand in candidates(mech, integrator):
+        ms = _time(cand, integrator, form, st, m, dt, rtol, atol, calls)
+        results.append(dict(cand, kernel_ms=ms))
+        log(f"[autotune] {mech} {integrator} {form} {cand}: {ms:.1f} ms")
+    best = min(results, key=lambda r: r["kernel_ms"])
+    entry = {"kernels": best["kernels"], "batched_waves_per_cu": best["batched_waves_per_cu"],
+             "kernel_ms": best["kernel_ms"], "cells": st.ncells, "dt": dt, "rtol": rtol, "atol": atol,
+             "candidates": results}
+    a = arch()
+    path = out or table_path(a)
+    table = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            table = json.load(f)
+    table[key(mech, integrator, form)] = entry
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    return entry
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--mech", default="drm19")
+    ap.add_argument("--integrator", default="ReactorBDFAdaptiveMixed")
+    ap.add_argument("--form", choices=["box", "1d"], default="box")
+    ap.add_argument("--cells", type=int, default=262144)
+    ap.add_argument("--dt", type=float, default=1e-5)
+    ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--atol", type=float, default=1e-16)
+    ap.add_argument("--calls", type=int, default=2)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    e = tune(a.mech, a.integrator, a.form, a.cells, a.dt, a.rtol, a.atol, a.calls, a.out,
+             log=lambda *x: print(*x, file=sys.stderr, flush=True))
+    print(json.dumps({k: v for k, v in e.items() if k != "candidates"}))
+
+
+if __name__ == "__main__":
+    main()

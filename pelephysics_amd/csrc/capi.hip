@@ -179,6 +179,8 @@ struct pp_chem {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   GraphKey gkey{};
+  // persistent-grid bound of the batched forms: batched_waves_per_cu x CUs
+  int max_blocks = 0;
 };
 
 namespace {
@@ -269,6 +271,7 @@ int prepare_workspace(pp_chem* h, int ncells) {
   int ncu = 0;
   HIPCHK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
   long long blocks = std::min<long long>((ncells + 63) / 64, 8LL * std::max(ncu, 1));
+  if (h->max_blocks > 0) blocks = std::min<long long>(blocks, h->max_blocks);
   const long long cap = h->cfg.workspace_bytes > 0 ? h->cfg.workspace_bytes : kWsMaxBytes;
   const long long lane_bytes = 64 * rows * (long long)sizeof(double);  // one 64-lane block
   if (lane_bytes > cap) return fail(h, PP_CHEM_EINVAL, "workspace_bytes below one 64-cell block");
@@ -289,7 +292,7 @@ int prepare_workspace(pp_chem* h, int ncells) {
 // Prepares the work queue: zero the counter and, when the previous call
 // integrated the same cells, order them by that call's RHS counts.
 ppc::QueueArgs prepare_queue(pp_chem* h, int ncells, const void* key, hipStream_t s) {
-  ppc::QueueArgs q{nullptr, h->d_queue, ncells};
+  ppc::QueueArgs q{nullptr, h->d_queue, ncells, h->max_blocks};
   hipLaunchKernelGGL(zero_ints, dim3(1), dim3(256), 0, s, h->d_queue, 1 + 2 * NBINS);
   if (h->hist_valid && h->hist_ncells == ncells && h->hist_key == key && ncells > 0) {
     int* bins = h->d_queue + 1;
@@ -450,6 +453,7 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
       (!(cfg->rtol > 0) || !(cfg->atol > 0) || cfg->max_steps < 1))
     return PP_CHEM_EINVAL;
   if (cfg->workspace_bytes < 0) return PP_CHEM_EINVAL;
+  if (cfg->batched_waves_per_cu < 0) return PP_CHEM_EINVAL;
   if (cfg->clean_init_massfrac != 0 && cfg->clean_init_massfrac != 1) return PP_CHEM_EINVAL;
   const int bs = cfg->block_size > 0 ? cfg->block_size : 64;
   if (bs > 64 || bs % 64 != 0) return PP_CHEM_EINVAL;  // kernels are built for 64-lane blocks
@@ -461,6 +465,11 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
   h->L = L;
   h->P = to_dev(*cfg);
   hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess && cfg->batched_waves_per_cu > 0) {
+    int ncu = 0;
+    e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device);
+    h->max_blocks = cfg->batched_waves_per_cu * ncu;  // one 64-lane wave per block
+  }
   if (e == hipSuccess) e = hipMalloc(&h->d_red, sizeof(unsigned long long) * 9);
   if (e == hipSuccess) e = hipMalloc(&h->d_queue, sizeof(int) * (1 + 2 * NBINS));
   if (e == hipSuccess) e = hipEventCreate(&h->ev0);

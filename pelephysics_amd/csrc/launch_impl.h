@@ -65,6 +65,21 @@ int resident_blocks(K kernel, long long lanes) {
   return (int)(blocks < need ? blocks : need);
 }
 
+// Persistent grid of a batched launch: every resident block, at most
+// q.max_blocks when set and at most the lanes the workspace holds (-1: no
+// device, -2: workspace missing).
+template <class M, int MODE, class K>
+int batched_grid(K kernel, const ReactParamsDev& P, const QueueArgs& q) {
+  int grid = resident_blocks(kernel, q.ncells);
+  if (grid < 0) return -1;
+  if (q.max_blocks > 0 && q.max_blocks < grid) grid = q.max_blocks;
+  if (integrator_ws_rows<M>(MODE) > 0) {
+    if (!P.ws || P.ws_lanes < 64 || P.ws_lanes % 64 != 0) return -2;
+    if (P.ws_lanes / 64 < grid) grid = (int)(P.ws_lanes / 64);
+  }
+  return grid;
+}
+
 template <class M, int MODE>
 hipError_t batched_1d_mode(const ReactParamsDev& P, double* y, const double* ysrc, double* rX, const double* rXsrc,
                            double dt, QueueArgs q, CellStats st, hipStream_t s) {
@@ -81,12 +96,8 @@ hipError_t batched_1d_mode(const ReactParamsDev& P, double* y, const double* ysr
 #endif
   {
     auto k = batched_kernel<M, LU_T, IO1D>;
-    int grid = resident_blocks(k, q.ncells);
-    if (grid < 0) return hipErrorInvalidDevice;
-    if (integrator_ws_rows<M>(MODE) > 0) {  // at most the lanes the workspace holds
-      if (!P.ws || P.ws_lanes < 64 || P.ws_lanes % 64 != 0) return hipErrorInvalidValue;
-      grid = (int)(grid < P.ws_lanes / 64 ? grid : P.ws_lanes / 64);
-    }
+    const int grid = batched_grid<M, MODE>(k, P, q);
+    if (grid < 0) return grid == -2 ? hipErrorInvalidValue : hipErrorInvalidDevice;
     IO1D io{y, ysrc, rX, rXsrc};
     hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
   }
@@ -109,12 +120,8 @@ hipError_t batched_box_mode(const ReactParamsDev& P, const BoxArgs& b, double dt
 #endif
   {
     auto k = batched_kernel<M, LU_T, IOBox>;
-    int grid = resident_blocks(k, q.ncells);
-    if (grid < 0) return hipErrorInvalidDevice;
-    if (integrator_ws_rows<M>(MODE) > 0) {  // at most the lanes the workspace holds
-      if (!P.ws || P.ws_lanes < 64 || P.ws_lanes % 64 != 0) return hipErrorInvalidValue;
-      grid = (int)(grid < P.ws_lanes / 64 ? grid : P.ws_lanes / 64);
-    }
+    const int grid = batched_grid<M, MODE>(k, P, q);
+    if (grid < 0) return grid == -2 ? hipErrorInvalidValue : hipErrorInvalidDevice;
     IOBox io{b};
     hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
   }

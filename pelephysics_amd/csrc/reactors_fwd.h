@@ -29,6 +29,7 @@ struct QueueArgs {
   const int* order;  // cell visiting order (null: identity)
   int* counter;      // global work counter (zeroed before the launch)
   int ncells;
+  int max_blocks;    // upper bound on the persistent grid (0: every resident block)
 };
 
 // Arguments of the discrete-adjoint kernel (wave_adjoint.h); device pointers.

@@ -64,7 +64,7 @@ class Config(C.Structure):
         ("rk64_nsubsteps_min", C.c_int), ("rk64_nsubsteps_max", C.c_int),
         ("rtol", C.c_double), ("atol", C.c_double), ("first_step", C.c_double),
         ("max_steps", C.c_int), ("block_size", C.c_int), ("workspace_bytes", C.c_longlong),
-        ("clean_init_massfrac", C.c_int),
+        ("clean_init_massfrac", C.c_int), ("batched_waves_per_cu", C.c_int),
     ]
 
 
@@ -95,18 +95,27 @@ _KEYS = {
     "rk64_nsubsteps_guess": "rk64_nsubsteps_guess", "rk64_nsubsteps_min": "rk64_nsubsteps_min",
     "rk64_nsubsteps_max": "rk64_nsubsteps_max", "mod_reactor": "mod_reactor",
     "block_size": "block_size", "workspace_bytes": "workspace_bytes",
-    "clean_init_massfrac": "clean_init_massfrac",
+    "clean_init_massfrac": "clean_init_massfrac", "batched_waves_per_cu": "batched_waves_per_cu",
 }
 
 _lib = None
 
 
 def library() -> C.CDLL:
-    """Load libpele_chem_hip.so (raises if it has not been built)."""
+    """Load libpele_chem_hip.so (raises if it has not been built).
+
+    torch is imported first: its HIP runtime is then the one the library
+    binds to, so the tensors handed to the library and the library's own
+    allocations live in one runtime (loaded the other way round, torch finds
+    no GPU)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: run python -m pelephysics_amd.build")
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp = C.c_void_p
         L.pp_chem_config_init.argtypes = [C.POINTER(Config), C.c_char_p, C.c_int, C.c_int]
@@ -184,9 +193,19 @@ class ReactorBase:
         self.reactor_type = 0
         self.mechanism = None
 
-    def init(self, reactor_type: int, ncells: int = 0, mechanism: str = "drm19", **ode) -> int:
-        """ReactorX::init(reactor_type, ncells); ode.* keys as keyword arguments."""
+    def init(self, reactor_type: int, ncells: int = 0, mechanism: str = "drm19", tuned: Optional[str] = None,
+             **ode) -> int:
+        """ReactorX::init(reactor_type, ncells); ode.* keys as keyword arguments.
+        tuned = "box" or "1d": take the kernel implementation and launch shape
+        the autotuner chose for this mechanism, integrator and form
+        (pelephysics_amd.autotune), when it has an entry."""
         L = library()
+        if tuned:
+            from .autotune import lookup
+            entry = lookup(mechanism, self.identifier, tuned)
+            if entry:
+                mechanism = entry["kernels"]
+                ode.setdefault("batched_waves_per_cu", entry["batched_waves_per_cu"])
         cfg = Config()
         rc = L.pp_chem_config_init(C.byref(cfg), mechanism.encode(), self._mode, reactor_type)
         if rc != 0:

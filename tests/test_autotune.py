@@ -1,0 +1,22 @@
+"""Autotuner bookkeeping (no GPU): candidate sets and the tuning table."""
+import json
+
+from pelephysics_amd import autotune
+
+
+def test_candidates():
+    c = autotune.candidates("drm19", "ReactorBDFAdaptiveMixed")
+    assert {(x["kernels"], x["batched_waves_per_cu"]) for x in c} == {
+        (k, w) for k in ("drm19", "drm19_table") for w in autotune.WAVES}
+    c = autotune.candidates("dodecane_lu", "ReactorBDFAdaptive")
+    assert c == [{"kernels": "dodecane_lu", "batched_waves_per_cu": 0}]
+
+
+def test_lookup_reads_the_table(tmp_path, monkeypatch):
+    p = tmp_path / "t.json"
+    p.write_text(json.dumps({"drm19|ReactorBDFAdaptiveMixed|box": {"kernels": "drm19_table",
+                                                                    "batched_waves_per_cu": 6}}))
+    monkeypatch.setenv("PELE_CHEM_TUNING", str(p))
+    e = autotune.lookup("drm19", "ReactorBDFAdaptiveMixed", "box")
+    assert e["kernels"] == "drm19_table" and e["batched_waves_per_cu"] == 6
+    assert autotune.lookup("drm19", "ReactorBDFAdaptiveMixed", "1d") is None
