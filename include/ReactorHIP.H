@@ -58,6 +58,8 @@ public:
     pp.query("rtol", m_cfg.rtol);
     pp.query("atol", m_cfg.atol);
     pp.query("max_steps", m_cfg.max_steps);
+    pp.query("batched_waves_per_cu", m_cfg.batched_waves_per_cu);  // autotuner output
+    pp.query("workspace_bytes", m_cfg.workspace_bytes);
     bool clean = false;
     pp.query("clean_init_massfrac", clean);  // ReactorBDF.cpp:127, ReactorRK64.cpp:18
     m_cfg.clean_init_massfrac = clean ? 1 : 0;
