@@ -37,14 +37,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(ident, st, m, steps, warmup, rtol, atol, mech=MECH, waves=0):
+TUNABLES = ("batched_waves_per_cu", "batched_lu_batch", "batched_lu_wait")
+
+
+def run_gpu(ident, st, m, steps, warmup, rtol, atol, mech=MECH, launch=None):
     import torch
     from pelephysics_amd.reactor import ReactorBase
     dev = torch.device("cuda:0")
     ns = m.n_species
     n = GRID
     R = ReactorBase.create(ident)
-    R.init(2, st.ncells, mechanism=mech, rtol=rtol, atol=atol, batched_waves_per_cu=waves)
+    R.init(2, st.ncells, mechanism=mech, rtol=rtol, atol=atol, **(launch or {}))
 
     def soa(a, nc):
         return torch.from_numpy(np.ascontiguousarray(a.T.reshape(nc, n, n, n))).to(dev)
@@ -85,7 +88,7 @@ def run_gpu(ident, st, m, steps, warmup, rtol, atol, mech=MECH, waves=0):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_drm19_64cubed.json")
 
 
-def measured_traffic(ident, kernels, waves):
+def measured_traffic(ident, kernels, launch):
     """HBM bytes per launch of the integration kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/), or None when they
     were taken on other kernels or another launch shape."""
@@ -95,7 +98,7 @@ def measured_traffic(ident, kernels, waves):
     except (OSError, ValueError):
         return None
     e = t.get(ident)
-    if e is None or e.get("kernels") != kernels or e.get("batched_waves_per_cu", 0) != waves:
+    if e is None or e.get("kernels") != kernels or e.get("launch", {k: 0 for k in TUNABLES}) != launch:
         return None
     return float(e["bytes_per_launch"])
 
@@ -232,23 +235,24 @@ def main():
              "fp64": "ReactorBDFAdaptive"}[a.mode]
     log(f"[bench] {ident} drm19 {GRID}^3 = {ncells} cells, warmup {a.warmup}, steps {a.steps}")
     def choose(kind, integrator):
-        """(kernels, waves per CU, tuning entry or None) of one measured form."""
+        """(kernels, launch parameters, tuning entry or None) of one measured form."""
         if kind == "tuned":
             from pelephysics_amd.autotune import lookup
             e = lookup(MECH, integrator, "box")
             if e:
-                return e["kernels"], e["batched_waves_per_cu"], {k: e[k] for k in e if k != "candidates"}
-            return MECH, 0, None
-        return {"unrolled": MECH, "table": MECH + "_table"}[kind], 0, None
+                return (e["kernels"], {k: e.get(k, 0) for k in TUNABLES},
+                        {k: e[k] for k in e if k != "candidates"})
+            return MECH, {k: 0 for k in TUNABLES}, None
+        return {"unrolled": MECH, "table": MECH + "_table"}[kind], {k: 0 for k in TUNABLES}, None
 
-    kern, waves, tuning = choose(a.impl, ident)
-    pkern, pwaves, ptuning = choose(a.plain_impl, "ReactorBDFAdaptive")
-    head = run_gpu(ident, st, m, a.steps, a.warmup, RTOL, ATOL, kern, waves)
+    kern, launch, tuning = choose(a.impl, ident)
+    pkern, plaunch, ptuning = choose(a.plain_impl, "ReactorBDFAdaptive")
+    head = run_gpu(ident, st, m, a.steps, a.warmup, RTOL, ATOL, kern, launch)
     value = ncells * a.steps / head["elapsed"]
     log(f"[bench] {ident}: {value:.4g} cells/s, kernel {head['kernel_ms']:.2f} ms")
     extra = {}
     if a.mode != "fp64" and not a.no_plain:
-        plain = run_gpu("ReactorBDFAdaptive", st, m, max(1, a.steps // 2), 1, RTOL, ATOL, pkern, pwaves)
+        plain = run_gpu("ReactorBDFAdaptive", st, m, max(1, a.steps // 2), 1, RTOL, ATOL, pkern, plaunch)
         pv = ncells * max(1, a.steps // 2) / plain["elapsed"]
         Yp = plain["rY"] / plain["rY"].sum(1, keepdims=True)
         Ym = head["rY"] / head["rY"].sum(1, keepdims=True)
@@ -259,7 +263,7 @@ def main():
         log(f"[bench] plain FP64: {pv:.4g} cells/s; speedup {value / pv:.3f}")
     s = head["stats"]
     rl = roofline(m, s, head["kernel_ms"], mixed)
-    rl["traffic"] = measured_traffic(ident, kern, waves)
+    rl["traffic"] = measured_traffic(ident, kern, launch)
     rl["algorithmic_bytes"] = (3 * m.n_species + 6) * 8 * ncells  # (3N+6)*8 B per cell: read+write once
     adj = None if a.no_adjoint else adjoint_measure(m)
     if adj:
@@ -276,7 +280,7 @@ def main():
         "config": {"workload": f"drm19 {GRID}^3 Gaussian-random-field mixture fraction x progress "
                                f"variable, h-reactor, dt=1e-5 s, adaptive BDF rtol={RTOL:g} "
                                f"atol={ATOL:g}",
-                   "integrator": ident, "kernels": kern, "batched_waves_per_cu": waves, "tuning": tuning,
+                   "integrator": ident, "kernels": kern, "launch": launch, "tuning": tuning,
                    "plain_kernels": pkern, "plain_tuning": ptuning,
                    "cells": ncells, "parallelism": "single GPU"},
         "per_cell": {k: s[k] / ncells for k in ("nsteps", "nrhs", "njac", "nlu", "nnewton", "nreject")},

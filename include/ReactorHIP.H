@@ -59,6 +59,8 @@ public:
     pp.query("atol", m_cfg.atol);
     pp.query("max_steps", m_cfg.max_steps);
     pp.query("batched_waves_per_cu", m_cfg.batched_waves_per_cu);  // autotuner output
+    pp.query("batched_lu_batch", m_cfg.batched_lu_batch);
+    pp.query("batched_lu_wait", m_cfg.batched_lu_wait);
     pp.query("workspace_bytes", m_cfg.workspace_bytes);
     bool clean = false;
     pp.query("clean_init_massfrac", clean);  // ReactorBDF.cpp:127, ReactorRK64.cpp:18

@@ -78,6 +78,10 @@ typedef struct {
    * 8 at the kernels' 2 waves per SIMD); a tuning parameter
    * (python -m pelephysics_amd.autotune). */
   int batched_waves_per_cu;
+  /* Batched forms: a wave factors the Newton matrices of its waiting lanes
+   * together once batched_lu_batch lanes (1..64) wait or a request has waited
+   * batched_lu_wait trips (0: 16 and 4); results do not depend on them. */
+  int batched_lu_batch, batched_lu_wait;
 } pp_chem_config;
 
 /* amrex::Array4<Real> layout: element (i,j,k,n) is at

@@ -12,7 +12,10 @@ keeps the fastest:
   ("<mech>_table"), where both exist;
 * batched_waves_per_cu (batched forms only): persistent waves per compute
   unit, 8 (every wave that fits at 2 waves per SIMD), 6 or 4.  Fewer waves
-  contend less for the caches that back private memory but hide less latency.
+  contend less for the caches that back private memory but hide less latency;
+* batched_lu_batch / batched_lu_wait (batched forms only): how many waiting
+  lanes, or how many trips of waiting, make a wave factor its lanes' Newton
+  matrices together (16 or 64 lanes, 4 or 8 trips).
 
 Each candidate runs `calls` calls on the same cells and arrays; its time is
 the kernel time of the last one (the queue in the cost order of the call
@@ -34,6 +37,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 BATCHED = ("ReactorBDFAdaptiveMixed", "ReactorBDFBatchedFP64")
 WAVES = (8, 6, 4)
+LU_SERVICE = ((16, 4), (64, 4), (16, 8), (64, 8))
 
 
 def arch() -> str:
@@ -62,8 +66,11 @@ def candidates(mech: str, integrator: str) -> list:
     from .reactor import mechanisms
     have = set(mechanisms())
     impls = [k for k in (mech, mech + "_table") if k in have]
-    waves = WAVES if integrator in BATCHED else (0,)
-    return [{"kernels": k, "batched_waves_per_cu": w} for k in impls for w in waves]
+    if integrator not in BATCHED:
+        return [{"kernels": k, "batched_waves_per_cu": 0, "batched_lu_batch": 0, "batched_lu_wait": 0}
+                for k in impls]
+    return [{"kernels": k, "batched_waves_per_cu": w, "batched_lu_batch": b, "batched_lu_wait": t}
+            for k in impls for w in WAVES for b, t in LU_SERVICE]
 
 
 def _time(cand: dict, integrator: str, form: str, st, m, dt: float, rtol: float, atol: float,
@@ -73,7 +80,7 @@ def _time(cand: dict, integrator: str, form: str, st, m, dt: float, rtol: float,
     dev = torch.device("cuda:0")
     R = ReactorBase.create(integrator)
     R.init(2, st.ncells, mechanism=cand["kernels"], rtol=rtol, atol=atol,
-           batched_waves_per_cu=cand["batched_waves_per_cu"])
+           **{k: v for k, v in cand.items() if k != "kernels"})
     ns = m.n_species
     ms = 0.0
     if form == "box":
@@ -120,9 +127,8 @@ def tune(mech: str, integrator: str, form: str = "box", ncells: int = 262144, dt
         results.append(dict(cand, kernel_ms=ms))
         log(f"[autotune] {mech} {integrator} {form} {cand}: {ms:.1f} ms")
     best = min(results, key=lambda r: r["kernel_ms"])
-    entry = {"kernels": best["kernels"], "batched_waves_per_cu": best["batched_waves_per_cu"],
-             "kernel_ms": best["kernel_ms"], "cells": st.ncells, "dt": dt, "rtol": rtol, "atol": atol,
-             "candidates": results}
+    entry = dict({k: best[k] for k in best if k != "kernel_ms"}, kernel_ms=best["kernel_ms"], cells=st.ncells,
+                 dt=dt, rtol=rtol, atol=atol, candidates=results)
     a = arch()
     path = out or table_path(a)
     table = {}

@@ -1,18 +1,28 @@
 // Batched adaptive BDF: persistent waves, a global work queue, and an
 // integrator written as a per-lane state machine around the right-hand side.
 //
-// Every trip through the main loop evaluates exactly one RHS for every lane
-// that still has a cell, then advances that lane's integrator (initial step
-// selection, modified-Newton iteration, error test, order/step selection,
-// Jacobian refresh, LU) until it needs the next RHS.  Lanes are therefore
-// never idle while a neighbour of the same wavefront is still integrating a
-// stiffer cell: when a lane finishes its cell it stores the result and takes
-// the next cell from the queue (atomic counter over a cost-ordered cell list).
+// Every trip through the main loop evaluates one RHS for every lane that
+// still has a cell and is not waiting for a factorisation (below), then
+// advances that lane's integrator (initial step selection, modified-Newton
+// iteration, error test, order/step selection, Jacobian and LU requests)
+// until it needs the next RHS.  Lanes are never idle while a neighbour of the
+// same wavefront is still integrating a stiffer cell: when a lane finishes its
+// cell it stores the result and takes the next cell from the queue (atomic
+// counter over a cost-ordered cell list).
 //
 // The step/order/Newton logic is the same NDF family as adaptive_cell
 // (reactors.h); the Newton matrix I - cJ is equilibrated with the error
 // weights, S^-1 (I - cJ) S, and factored in LU_T (float: FP32 LU with FP64
 // residuals, rates, T inversion and error control).
+//
+// A wave executes a Jacobian or an LU factorisation whenever any of its lanes
+// needs one, so with one request per lane every ~10 trips nearly every trip
+// would pay for a factorisation.  Lanes therefore post Jacobian / LU requests
+// and the wave serves them together, at one place in the trip, once
+// lu_batch lanes wait, every remaining lane waits, or a request has waited
+// lu_wait trips; a waiting lane skips its trips (no RHS, no Newton
+// iteration) until it is served.  The arithmetic of every cell is unchanged,
+// so results do not depend on the batching.
 #pragma once
 #include "reactors.h"
 
@@ -113,6 +123,11 @@ __global__ void __launch_bounds__(64) PPC_LANE_OCC batched_kernel(ReactParamsDev
   int order = 1, n_equal = 0, k_newton = 0, phase = PH_F0;
   bool lu_valid = false, current_jac = false;
   int nsteps = 0, nrhs = 0, njac = 0, nlu = 0, nnewton = 0, nrej = 0, status = 0;
+  // pending Jacobian (at yjac) and LU requests, trips waited
+  bool need_jac = false, need_lu = false;
+  int wait = 0;
+  double yjac[n];
+  const int lu_batch = ap.lu_batch > 0 ? ap.lu_batch : 16, lu_wait = ap.lu_wait > 0 ? ap.lu_wait : 4;
 
   int cell = 0, cid = 0;
   bool active = false;
@@ -126,6 +141,8 @@ __global__ void __launch_bounds__(64) PPC_LANE_OCC batched_kernel(ReactParamsDev
     order = 1;
     n_equal = 0;
     lu_valid = false;
+    need_jac = need_lu = false;
+    wait = 0;
     nsteps = nrhs = njac = nlu = nnewton = nrej = status = 0;
   };
 
@@ -186,6 +203,11 @@ __global__ void __launch_bounds__(64) PPC_LANE_OCC batched_kernel(ReactParamsDev
     njac++;
   };
 
+  auto request_jacobian = [&](const double* yj) {
+    for (int i = 0; i < n; ++i) yjac[i] = yj[i];
+    need_jac = true;
+  };
+
   // top of one step attempt: predictor, weights, LU if needed
   auto begin_attempt = [&]() -> bool {
     const double min_step = 10 * fabs(nextafter(t, INFINITY) - t);
@@ -212,7 +234,7 @@ __global__ void __launch_bounds__(64) PPC_LANE_OCC batched_kernel(ReactParamsDev
       psi[i] = ps * inv_alpha;
     }
     c = h * inv_alpha;
-    if (!lu_valid) factor();
+    if (!lu_valid) need_lu = true;
     for (int i = 0; i < n; ++i) {
       yv[i] = ypred[i];
       d[i] = 0.0;
@@ -239,6 +261,27 @@ __global__ void __launch_bounds__(64) PPC_LANE_OCC batched_kernel(ReactParamsDev
   };
 
   while (active) {
+    // ---- Jacobian / LU requests, served for the waiting lanes together
+    const unsigned long long waiting = __ballot(need_lu);
+    if (waiting) {
+      const unsigned long long all = __ballot(true);
+      const bool stale = __ballot(need_lu && wait >= lu_wait) != 0;
+      if (__popcll(waiting) >= lu_batch || waiting == all || stale) {
+        if (need_jac) {
+          jacobian(yjac);
+          need_jac = false;
+        }
+        if (need_lu) {
+          factor();
+          need_lu = false;
+          wait = 0;
+        }
+      }
+    }
+    if (need_lu) {  // not served yet: no RHS, no Newton iteration this trip
+      ++wait;
+      continue;
+    }
     // ---- the one RHS evaluation of this trip
     const double t_eval = (phase == PH_F0) ? 0.0 : (phase == PH_F1 ? h0 : t_new);
     f_kernel_spec<M>(t_eval, P.reactor_type, yv, f, rhoe, rhoesrc, src);
@@ -254,7 +297,7 @@ __global__ void __launch_bounds__(64) PPC_LANE_OCC batched_kernel(ReactParamsDev
         h_abs = ap.first_step;
         for (int i = 0; i < n; ++i) D[n + i] *= h_abs;
         for (int i = 0; i < n; ++i) yv[i] = D[i];
-        jacobian(yv);
+        request_jacobian(yv);
         ok = begin_step();
       } else {
         const double d0 = rms_norm<n>(D, scl), d1 = rms_norm<n>(f, scl);
@@ -271,7 +314,7 @@ __global__ void __launch_bounds__(64) PPC_LANE_OCC batched_kernel(ReactParamsDev
       h_abs = fmin(fmin(100 * h0, h1), t_bound);
       for (int i = 0; i < n; ++i) D[n + i] *= h_abs;
       for (int i = 0; i < n; ++i) yv[i] = D[i];
-      jacobian(yv);
+      request_jacobian(yv);
       ok = begin_step();
     } else {
       // one modified-Newton iteration with f = f(t_new, yv)
@@ -306,9 +349,9 @@ __global__ void __launch_bounds__(64) PPC_LANE_OCC batched_kernel(ReactParamsDev
       if (failed) {
         if (!current_jac) {
           // refresh the Jacobian at the predictor and retry the same step
-          jacobian(ypred);
+          request_jacobian(ypred);
           current_jac = true;
-          factor();
+          need_lu = true;
           for (int i = 0; i < n; ++i) {
             yv[i] = ypred[i];
             d[i] = 0.0;

@@ -334,7 +334,7 @@ ppc::ReactParamsDev to_dev(const pp_chem_config& c) {
   P.bdf = {c.bdf_nsubsteps, c.bdf_nonlinear_iters, c.bdf_gmres_restarts, c.bdf_gmres_kspiters,
            c.bdf_gmres_precond, c.bdf_scheme, c.bdf_nonlinear_tol, c.bdf_gmres_tol};
   P.rk64 = {c.rk64_atol, c.rk64_nsubsteps_guess, c.rk64_nsubsteps_min, c.rk64_nsubsteps_max};
-  P.adapt = {c.rtol, c.atol, c.first_step, c.max_steps};
+  P.adapt = {c.rtol, c.atol, c.first_step, c.max_steps, c.batched_lu_batch, c.batched_lu_wait};
   return P;
 }
 
@@ -453,7 +453,9 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
       (!(cfg->rtol > 0) || !(cfg->atol > 0) || cfg->max_steps < 1))
     return PP_CHEM_EINVAL;
   if (cfg->workspace_bytes < 0) return PP_CHEM_EINVAL;
-  if (cfg->batched_waves_per_cu < 0) return PP_CHEM_EINVAL;
+  if (cfg->batched_waves_per_cu < 0 || cfg->batched_lu_batch < 0 || cfg->batched_lu_batch > 64 ||
+      cfg->batched_lu_wait < 0)
+    return PP_CHEM_EINVAL;
   if (cfg->clean_init_massfrac != 0 && cfg->clean_init_massfrac != 1) return PP_CHEM_EINVAL;
   const int bs = cfg->block_size > 0 ? cfg->block_size : 64;
   if (bs > 64 || bs % 64 != 0) return PP_CHEM_EINVAL;  // kernels are built for 64-lane blocks

@@ -65,6 +65,7 @@ class Config(C.Structure):
         ("rtol", C.c_double), ("atol", C.c_double), ("first_step", C.c_double),
         ("max_steps", C.c_int), ("block_size", C.c_int), ("workspace_bytes", C.c_longlong),
         ("clean_init_massfrac", C.c_int), ("batched_waves_per_cu", C.c_int),
+        ("batched_lu_batch", C.c_int), ("batched_lu_wait", C.c_int),
     ]
 
 
@@ -96,6 +97,7 @@ _KEYS = {
     "rk64_nsubsteps_max": "rk64_nsubsteps_max", "mod_reactor": "mod_reactor",
     "block_size": "block_size", "workspace_bytes": "workspace_bytes",
     "clean_init_massfrac": "clean_init_massfrac", "batched_waves_per_cu": "batched_waves_per_cu",
+    "batched_lu_batch": "batched_lu_batch", "batched_lu_wait": "batched_lu_wait",
 }
 
 _lib = None
@@ -205,7 +207,9 @@ class ReactorBase:
             entry = lookup(mechanism, self.identifier, tuned)
             if entry:
                 mechanism = entry["kernels"]
-                ode.setdefault("batched_waves_per_cu", entry["batched_waves_per_cu"])
+                for k in ("batched_waves_per_cu", "batched_lu_batch", "batched_lu_wait"):
+                    if k in entry:
+                        ode.setdefault(k, entry[k])
         cfg = Config()
         rc = L.pp_chem_config_init(C.byref(cfg), mechanism.encode(), self._mode, reactor_type)
         if rc != 0:
