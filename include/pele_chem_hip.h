@@ -82,6 +82,10 @@ typedef struct {
    * together once batched_lu_batch lanes (1..64) wait or a request has waited
    * batched_lu_wait trips (0: 16 and 4); results do not depend on them. */
   int batched_lu_batch, batched_lu_wait;
+  /* Batched forms, box call: 0 (default) runs box -> flat, the 1-D kernel,
+   * flat -> box (same per-cell operations, faster kernel build); 1 runs the
+   * box kernel directly. */
+  int batched_box_direct;
 } pp_chem_config;
 
 /* amrex::Array4<Real> layout: element (i,j,k,n) is at

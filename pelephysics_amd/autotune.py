@@ -15,7 +15,9 @@ keeps the fastest:
   contend less for the caches that back private memory but hide less latency;
 * batched_lu_batch / batched_lu_wait (batched forms only): how many waiting
   lanes, or how many trips of waiting, make a wave factor its lanes' Newton
-  matrices together (16 or 64 lanes, 4 or 8 trips).
+  matrices together (16 or 64 lanes, 4 or 8 trips);
+* batched_box_direct (batched forms, box calls): the box kernel itself (1) or
+  box -> flat state, the 1-D kernel, flat -> box (0).
 
 Each candidate runs `calls` calls on the same cells and arrays; its time is
 the kernel time of the last one (the queue in the cost order of the call
@@ -62,15 +64,17 @@ def lookup(mech: str, integrator: str, form: str = "box", a: str | None = None) 
         return json.load(f).get(key(mech, integrator, form))
 
 
-def candidates(mech: str, integrator: str) -> list:
+def candidates(mech: str, integrator: str, form: str = "1d") -> list:
     from .reactor import mechanisms
     have = set(mechanisms())
     impls = [k for k in (mech, mech + "_table") if k in have]
     if integrator not in BATCHED:
-        return [{"kernels": k, "batched_waves_per_cu": 0, "batched_lu_batch": 0, "batched_lu_wait": 0}
-                for k in impls]
-    return [{"kernels": k, "batched_waves_per_cu": w, "batched_lu_batch": b, "batched_lu_wait": t}
-            for k in impls for w in WAVES for b, t in LU_SERVICE]
+        return [{"kernels": k, "batched_waves_per_cu": 0, "batched_lu_batch": 0, "batched_lu_wait": 0,
+                 "batched_box_direct": 0} for k in impls]
+    routes = (0, 1) if form == "box" else (0,)
+    return [{"kernels": k, "batched_waves_per_cu": w, "batched_lu_batch": b, "batched_lu_wait": t,
+             "batched_box_direct": r}
+            for k in impls for w in WAVES for b, t in LU_SERVICE for r in routes]
 
 
 def _time(cand: dict, integrator: str, form: str, st, m, dt: float, rtol: float, atol: float,
@@ -122,7 +126,7 @@ def tune(mech: str, integrator: str, form: str = "box", ncells: int = 262144, dt
     st = workload(mech, ncells)
     m = load(mech)
     results = []
-    for cand in candidates(mech, integrator):
+    for cand in candidates(mech, integrator, form):
         ms = _time(cand, integrator, form, st, m, dt, rtol, atol, calls)
         results.append(dict(cand, kernel_ms=ms))
         log(f"[autotune] {mech} {integrator} {form} {cand}: {ms:.1f} ms")

@@ -457,6 +457,7 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
       cfg->batched_lu_wait < 0)
     return PP_CHEM_EINVAL;
   if (cfg->clean_init_massfrac != 0 && cfg->clean_init_massfrac != 1) return PP_CHEM_EINVAL;
+  if (cfg->batched_box_direct != 0 && cfg->batched_box_direct != 1) return PP_CHEM_EINVAL;
   const int bs = cfg->block_size > 0 ? cfg->block_size : 64;
   if (bs > 64 || bs % 64 != 0) return PP_CHEM_EINVAL;  // kernels are built for 64-lane blocks
   pp_chem* h = new pp_chem();
@@ -563,7 +564,39 @@ int pp_chem_react_box(pp_chem* h, const int lo[3], const int hi[3], const pp_arr
   HIPCHK(h, hipEventRecord(h->ev0, s));
   if (ncells > 0) {
     hipError_t e;
-    if (is_batched(h->cfg.mode)) {
+    if (is_batched(h->cfg.mode) && !h->cfg.batched_box_direct) {
+      // flat state, the 1-D kernel, back to the box: the same operations per
+      // cell as the box kernel (T from e / rho with the box T as guess;
+      // rE + dt rEsrc; T from the new energy with the integrated T as guess;
+      // FC = steps), and the 1-D kernel is the faster build of the two
+      const int ns = h->L->ns;
+      const size_t nc = (size_t)ncells, need = nc * (size_t)(2 * ns + 3);
+      if (need > h->stage_cap) {
+        if (h->d_stage) HIPCHK(h, hipFree(h->d_stage));
+        h->d_stage = nullptr;
+        h->stage_cap = 0;
+        HIPCHK(h, hipMalloc(&h->d_stage, need * sizeof(double)));
+        h->stage_cap = need;
+      }
+      ppc::FlatArgs fa{};
+      fa.b = b;
+      fa.y = h->d_stage;
+      fa.ysrc = fa.y + nc * (ns + 1);
+      fa.e = fa.ysrc + nc * ns;
+      fa.esrc = fa.e + nc;
+      fa.reactor_type = h->cfg.reactor_type;
+      fa.divide = 1;
+      e = h->L->flatten(fa, 0, s);
+      if (e == hipSuccess) {
+        const ppc::QueueArgs q = prepare_queue(h, ncells, rY->p, s);
+        e = h->L->batched_1d(h->cfg.mode, h->P, fa.y, fa.ysrc, fa.e, fa.esrc, *dt_react, q, counters(h), s);
+      }
+      if (e == hipSuccess) {
+        fa.dt = 0.0;  // the 1-D kernel has already advanced e by dt * rEsrc
+        fa.fc32 = counters(h).nsteps;
+        e = h->L->flatten(fa, 1, s);
+      }
+    } else if (is_batched(h->cfg.mode)) {
       const ppc::QueueArgs q = prepare_queue(h, ncells, rY->p, s);
       e = h->L->batched_box(h->cfg.mode, h->P, b, *dt_react, q, counters(h), s);
     } else {

@@ -13,6 +13,9 @@
 //             with the flat T as the guess; FC = fc[c].
 //
 // Cell c of the box [lo, lo + len) is c = (k - lo_z) len_x len_y + (j - lo_y) len_x + (i - lo_x).
+// With a.divide the energy per unit mass is e / rho, as ReactorBDF's own box
+// path forms it (ReactorBDF.cpp:379,469), instead of FlattenOps' e * (1 / rho):
+// the batched box form runs through these kernels and the 1-D kernel.
 #pragma once
 #include "chem_cell.h"
 #include "reactors_fwd.h"
@@ -39,11 +42,12 @@ __global__ void __launch_bounds__(256) flatten_kernel(FlatArgs a) {
       a.ysrc[(long long)c * NS + n] = b.rYsrc(i, j, k, n);
     }
     double rinv;
-    rho_and_Y<M>(rY, Y, rinv);
+    const double rho = rho_and_Y<M>(rY, Y, rinv);
     const double e = b.rE(i, j, k, 0);
     a.e[c] = e;
     a.esrc[c] = b.rEsrc(i, j, k, 0);
-    a.y[(long long)c * (NS + 1) + NS] = T_from_energy_rt<M>(a.reactor_type, e * rinv, Y, b.T(i, j, k, 0));
+    a.y[(long long)c * (NS + 1) + NS] =
+        T_from_energy_rt<M>(a.reactor_type, a.divide ? e / rho : e * rinv, Y, b.T(i, j, k, 0));
   }
 }
 
@@ -71,11 +75,11 @@ __global__ void __launch_bounds__(256) unflatten_kernel(FlatArgs a) {
     }
     for (int n = 0; n < NS; ++n) b.rY(i, j, k, n) = rY[n];
     double rinv;
-    rho_and_Y<M>(rY, Y, rinv);
+    const double rho = rho_and_Y<M>(rY, Y, rinv);
     const double e = a.e[c] + a.dt * b.rEsrc(i, j, k, 0);
     b.rE(i, j, k, 0) = e;
-    b.T(i, j, k, 0) = T_from_energy_rt<M>(a.reactor_type, e * rinv, Y, yc[NS]);
-    if (b.FC.p) b.FC(i, j, k, 0) = (double)a.fc[c];
+    b.T(i, j, k, 0) = T_from_energy_rt<M>(a.reactor_type, a.divide ? e / rho : e * rinv, Y, yc[NS]);
+    if (b.FC.p) b.FC(i, j, k, 0) = a.fc ? (double)a.fc[c] : (double)a.fc32[c];
   }
 }
 

@@ -51,8 +51,10 @@ struct FlatArgs {
   double* e;             // ncells: rho e or rho h
   double* esrc;          // ncells (flatten only)
   const long long* fc;   // ncells step counts written to FC (unflatten only)
+  const int* fc32;       // the same as int counts, used when fc is null
   double dt;             // unflatten: rE = e + dt * rEsrc
   int reactor_type, clean;
+  int divide;            // energy per mass as e / rho (ReactorBDF.cpp:379,469) instead of e * (1 / rho)
 };
 
 struct CellStats {

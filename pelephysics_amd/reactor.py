@@ -65,7 +65,7 @@ class Config(C.Structure):
         ("rtol", C.c_double), ("atol", C.c_double), ("first_step", C.c_double),
         ("max_steps", C.c_int), ("block_size", C.c_int), ("workspace_bytes", C.c_longlong),
         ("clean_init_massfrac", C.c_int), ("batched_waves_per_cu", C.c_int),
-        ("batched_lu_batch", C.c_int), ("batched_lu_wait", C.c_int),
+        ("batched_lu_batch", C.c_int), ("batched_lu_wait", C.c_int), ("batched_box_direct", C.c_int),
     ]
 
 
@@ -98,6 +98,7 @@ _KEYS = {
     "block_size": "block_size", "workspace_bytes": "workspace_bytes",
     "clean_init_massfrac": "clean_init_massfrac", "batched_waves_per_cu": "batched_waves_per_cu",
     "batched_lu_batch": "batched_lu_batch", "batched_lu_wait": "batched_lu_wait",
+    "batched_box_direct": "batched_box_direct",
 }
 
 _lib = None
@@ -207,7 +208,7 @@ class ReactorBase:
             entry = lookup(mechanism, self.identifier, tuned)
             if entry:
                 mechanism = entry["kernels"]
-                for k in ("batched_waves_per_cu", "batched_lu_batch", "batched_lu_wait"):
+                for k in ("batched_waves_per_cu", "batched_lu_batch", "batched_lu_wait", "batched_box_direct"):
                     if k in entry:
                         ode.setdefault(k, entry[k])
         cfg = Config()

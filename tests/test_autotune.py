@@ -5,12 +5,13 @@ from pelephysics_amd import autotune
 
 
 def test_candidates():
-    c = autotune.candidates("drm19", "ReactorBDFAdaptiveMixed")
+    c = autotune.candidates("drm19", "ReactorBDFAdaptiveMixed", "1d")
     assert {(x["kernels"], x["batched_waves_per_cu"], x["batched_lu_batch"], x["batched_lu_wait"]) for x in c} == {
         (k, w, b, t) for k in ("drm19", "drm19_table") for w in autotune.WAVES for b, t in autotune.LU_SERVICE}
+    assert len(autotune.candidates("drm19", "ReactorBDFAdaptiveMixed", "box")) == 2 * len(c)
     c = autotune.candidates("dodecane_lu", "ReactorBDFAdaptive")
     assert c == [{"kernels": "dodecane_lu", "batched_waves_per_cu": 0, "batched_lu_batch": 0,
-                  "batched_lu_wait": 0}]
+                  "batched_lu_wait": 0, "batched_box_direct": 0}]
 
 
 def test_lookup_reads_the_table(tmp_path, monkeypatch):

@@ -399,3 +399,32 @@ def test_full_size_invariants_drm19_64cubed(models, ident):
     assert np.abs(h - st.rE).max() / np.abs(st.rE).max() < 1e-6
     assert R.stats()["nfailed"] == 0
     assert (y[:, -1] >= 299.0).all() and (y[:, -1] < 3000).all()
+
+
+@pytest.mark.parametrize("ident", ["ReactorBDFAdaptiveMixed", "ReactorBDFBatchedFP64"])
+def test_batched_box_routes_agree(models, ident):
+    """The batched box call through flat state and the 1-D kernel (default)
+    and through the box kernel give the same rY, T, rE and FC, bit for bit."""
+    torch = _torch()
+    from pelephysics_amd.reactor import ReactorBase
+    m = models["drm19"]
+    st = W.drm19_grf(m, 8)
+    n = 8
+    dev = torch.device("cuda:0")
+
+    def soa(a, nc):
+        return torch.from_numpy(np.ascontiguousarray(a.T.reshape(nc, n, n, n))).to(dev)
+
+    rng = np.random.default_rng(3)
+    rYs = soa(1e-3 * rng.standard_normal(st.rY.shape) * st.rY, m.n_species)
+    rEs = soa((1e-3 * rng.standard_normal(st.ncells) * np.abs(st.rE))[:, None], 1)
+    out = []
+    for direct in (0, 1):
+        R = ReactorBase.create(ident)
+        R.init(2, st.ncells, mechanism="drm19", rtol=1e-10, atol=1e-16, batched_box_direct=direct)
+        rY, T, rE = soa(st.rY, m.n_species), soa(st.T[:, None] + 3.0, 1), soa(st.rE[:, None], 1)
+        FC = torch.zeros_like(T)
+        R.react(rY, rYs, T, rE, rEs, FC, 1e-5)
+        out.append([x.cpu().numpy() for x in (rY, T, rE, FC)])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
