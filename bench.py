@@ -37,7 +37,8 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-TUNABLES = ("batched_waves_per_cu", "batched_lu_batch", "batched_lu_wait", "batched_box_direct")
+TUNABLES = ("batched_waves_per_cu", "batched_lu_batch", "batched_lu_wait", "batched_box_direct",
+            "batched_waves_per_simd")
 
 
 def run_gpu(ident, st, m, steps, warmup, rtol, atol, mech=MECH, launch=None):

@@ -61,6 +61,8 @@ public:
     pp.query("batched_waves_per_cu", m_cfg.batched_waves_per_cu);  // autotuner output
     pp.query("batched_lu_batch", m_cfg.batched_lu_batch);
     pp.query("batched_lu_wait", m_cfg.batched_lu_wait);
+    pp.query("batched_waves_per_simd", m_cfg.batched_waves_per_simd);
+    pp.query("batched_box_direct", m_cfg.batched_box_direct);
     pp.query("workspace_bytes", m_cfg.workspace_bytes);
     bool clean = false;
     pp.query("clean_init_massfrac", clean);  // ReactorBDF.cpp:127, ReactorRK64.cpp:18

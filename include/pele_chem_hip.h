@@ -86,6 +86,10 @@ typedef struct {
    * flat -> box (same per-cell operations, faster kernel build); 1 runs the
    * box kernel directly. */
   int batched_box_direct;
+  /* Batched forms: the kernel build held to 2 waves per SIMD (0 or 2, the
+   * default) or to 1 (more registers per lane, fewer spills; 1-D kernel only,
+   * so not with batched_box_direct = 1). */
+  int batched_waves_per_simd;
 } pp_chem_config;
 
 /* amrex::Array4<Real> layout: element (i,j,k,n) is at

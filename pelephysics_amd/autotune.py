@@ -10,18 +10,23 @@ keeps the fastest:
 * kernels: the compiled implementations of the mechanism, the generated
   straight-line code ("<mech>") and the constant-memory tables
   ("<mech>_table"), where both exist;
-* batched_waves_per_cu (batched forms only): persistent waves per compute
-  unit, 8 (every wave that fits at 2 waves per SIMD), 6 or 4.  Fewer waves
-  contend less for the caches that back private memory but hide less latency;
+* batched_waves_per_simd / batched_waves_per_cu (batched forms only): the
+  kernel build held to 2 waves per SIMD (8, 6 or 4 persistent waves per
+  compute unit) or to 1 (4 or 3 per CU).  Fewer waves contend less for the
+  caches that back private memory but hide less latency; the 1-wave build
+  spills less;
 * batched_lu_batch / batched_lu_wait (batched forms only): how many waiting
   lanes, or how many trips of waiting, make a wave factor its lanes' Newton
-  matrices together (16 or 64 lanes, 4 or 8 trips);
+  matrices together (16 lanes or 4 trips, 64 lanes or 8 trips);
 * batched_box_direct (batched forms, box calls): the box kernel itself (1) or
   box -> flat state, the 1-D kernel, flat -> box (0).
 
 Each candidate runs `calls` calls on the same cells and arrays; its time is
 the kernel time of the last one (the queue in the cost order of the call
-before).  Results go to a JSON table
+before).  A candidate is eligible only if every cell integrates and its
+result agrees with the first candidate's to |x - x_ref| <= 1e-6 |x_ref| +
+1e-10 on mass fractions and T (the launch shapes do not change any cell's
+arithmetic, so a disagreement means a faulty build).  Results go to a JSON table
 keyed by the GPU architecture (gcnArchName) and "<mech>|<integrator>|<form>";
 ``ReactorBase.init(..., tuned="box" or "1d")`` and bench.py read it back
 (pelephysics_amd/tuning/<arch>.json unless PELE_CHEM_TUNING names another
@@ -38,8 +43,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 BATCHED = ("ReactorBDFAdaptiveMixed", "ReactorBDFBatchedFP64")
-WAVES = (8, 6, 4)
-LU_SERVICE = ((16, 4), (64, 4), (16, 8), (64, 8))
+OCCUPANCY = ((2, 8), (2, 6), (2, 4), (1, 4), (1, 3))  # (waves per SIMD, waves per CU)
+LU_SERVICE = ((16, 4), (64, 8))
 
 
 def arch() -> str:
@@ -69,16 +74,18 @@ def candidates(mech: str, integrator: str, form: str = "1d") -> list:
     have = set(mechanisms())
     impls = [k for k in (mech, mech + "_table") if k in have]
     if integrator not in BATCHED:
-        return [{"kernels": k, "batched_waves_per_cu": 0, "batched_lu_batch": 0, "batched_lu_wait": 0,
-                 "batched_box_direct": 0} for k in impls]
+        return [{"kernels": k, "batched_waves_per_simd": 0, "batched_waves_per_cu": 0, "batched_lu_batch": 0,
+                 "batched_lu_wait": 0, "batched_box_direct": 0} for k in impls]
     routes = (0, 1) if form == "box" else (0,)
-    return [{"kernels": k, "batched_waves_per_cu": w, "batched_lu_batch": b, "batched_lu_wait": t,
-             "batched_box_direct": r}
-            for k in impls for w in WAVES for b, t in LU_SERVICE for r in routes]
+    return [{"kernels": k, "batched_waves_per_simd": o, "batched_waves_per_cu": w, "batched_lu_batch": b,
+             "batched_lu_wait": t, "batched_box_direct": r}
+            for k in impls for o, w in OCCUPANCY for b, t in LU_SERVICE for r in routes
+            if not (o == 1 and r == 1)]  # the box kernel is built at 2 waves per SIMD only
 
 
 def _time(cand: dict, integrator: str, form: str, st, m, dt: float, rtol: float, atol: float,
-          calls: int) -> float:
+          calls: int):
+    """(kernel ms of the last call, final state (ncells, N+1) as T and mass fractions)."""
     import torch
     from .reactor import ReactorBase
     dev = torch.device("cuda:0")
@@ -103,6 +110,7 @@ def _time(cand: dict, integrator: str, form: str, st, m, dt: float, rtol: float,
             rE.copy_(rE0)
             R.react(rY, rYs, T, rE, rEs, FC, dt)
             ms = R.stats()["kernel_ms"]
+        out = np.concatenate([rY.reshape(ns, -1).T.cpu().numpy(), T.reshape(-1, 1).cpu().numpy()], axis=1)
     else:
         y0 = torch.from_numpy(st.y_1d()).to(dev)
         x0 = torch.from_numpy(st.rE.copy()).to(dev)
@@ -114,8 +122,15 @@ def _time(cand: dict, integrator: str, form: str, st, m, dt: float, rtol: float,
             x.copy_(x0)
             R.react_1d(y, ys, x, xs, dt)
             ms = R.stats()["kernel_ms"]
+        out = y.reshape(st.ncells, ns + 1).cpu().numpy()
     R.close()
-    return ms
+    Y = out[:, :ns] / out[:, :ns].sum(1, keepdims=True)
+    return ms, np.concatenate([Y, out[:, ns:]], axis=1)
+
+
+def _agrees(a, b) -> bool:
+    """Per-component |a - b| <= 1e-6 |b| + 1e-10 on mass fractions and T."""
+    return bool(np.all(np.abs(a - b) <= 1e-6 * np.abs(b) + 1e-10))
 
 
 def tune(mech: str, integrator: str, form: str = "box", ncells: int = 262144, dt: float = 1e-5,
@@ -125,12 +140,27 @@ def tune(mech: str, integrator: str, form: str = "box", ncells: int = 262144, dt
     from .throughput import workload
     st = workload(mech, ncells)
     m = load(mech)
-    results = []
+    from .reactor import ReactorError
+    results, ref = [], None
     for cand in candidates(mech, integrator, form):
-        ms = _time(cand, integrator, form, st, m, dt, rtol, atol, calls)
-        results.append(dict(cand, kernel_ms=ms))
-        log(f"[autotune] {mech} {integrator} {form} {cand}: {ms:.1f} ms")
-    best = min(results, key=lambda r: r["kernel_ms"])
+        # every launch shape integrates each cell with the same arithmetic: a
+        # candidate whose cells fail or whose result leaves the criterion
+        # around the first candidate's is not eligible
+        try:
+            ms, res = _time(cand, integrator, form, st, m, dt, rtol, atol, calls)
+        except ReactorError as e:
+            results.append(dict(cand, kernel_ms=None, rejected=str(e)))
+            log(f"[autotune] {mech} {integrator} {form} {cand}: rejected ({e})")
+            continue
+        if ref is None:
+            ref = res
+        ok = _agrees(res, ref)
+        results.append(dict(cand, kernel_ms=ms) if ok else dict(cand, kernel_ms=None, rejected="result differs"))
+        log(f"[autotune] {mech} {integrator} {form} {cand}: {ms:.1f} ms" + ("" if ok else " rejected: result differs"))
+    eligible = [r for r in results if r["kernel_ms"] is not None]
+    if not eligible:
+        raise RuntimeError(f"no candidate of {mech} {integrator} {form} integrated every cell")
+    best = min(eligible, key=lambda r: r["kernel_ms"])
     entry = dict({k: best[k] for k in best if k != "kernel_ms"}, kernel_ms=best["kernel_ms"], cells=st.ncells,
                  dt=dt, rtol=rtol, atol=atol, candidates=results)
     a = arch()

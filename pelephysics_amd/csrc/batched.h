@@ -88,16 +88,14 @@ struct IOBox {
 
 enum : int { PH_F0 = 0, PH_F1 = 1, PH_NEWTON = 2 };
 
-// Waves per SIMD the register allocation of the batched kernel is held to:
-// at 2 (<= 256 VGPRs) the spills grow but the second wave hides the private-
-// memory latency that dominates this kernel; drm19 64^3: 616 ms vs 780 ms at
-// the compiler's default (1 wave, 256 VGPR + AGPRs) and 654 ms at 4.
-#ifndef PPC_LANE_OCC
-#define PPC_LANE_OCC __attribute__((amdgpu_waves_per_eu(2)))
-#endif
-
-template <class M, typename LU_T, class IO>
-__global__ void __launch_bounds__(64) PPC_LANE_OCC batched_kernel(ReactParamsDev P, IO io, double dt, QueueArgs q,
+// OCC: waves per SIMD the register allocation is held to.  At 2 (256 VGPRs)
+// more of the per-lane state spills but a second wave hides private-memory
+// latency; at 1 (256 VGPRs + 256 AGPRs) a third of the spills remain.  Which
+// wins depends on the mechanism and on how often the lanes factor (drm19
+// 64^3, 1-D form: 277 ms at 2, 231 ms at 1); both are compiled and the
+// launch picks one (pp_chem_config.batched_waves_per_simd).
+template <class M, typename LU_T, class IO, int OCC>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, OCC == 1 ? 1 : 8))) batched_kernel(ReactParamsDev P, IO io, double dt, QueueArgs q,
                                                      CellStats st) {
   constexpr int n = M::NEQ, NS = M::NS;
   const AdaptParamsDev& ap = P.adapt;

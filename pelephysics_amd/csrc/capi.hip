@@ -334,7 +334,8 @@ ppc::ReactParamsDev to_dev(const pp_chem_config& c) {
   P.bdf = {c.bdf_nsubsteps, c.bdf_nonlinear_iters, c.bdf_gmres_restarts, c.bdf_gmres_kspiters,
            c.bdf_gmres_precond, c.bdf_scheme, c.bdf_nonlinear_tol, c.bdf_gmres_tol};
   P.rk64 = {c.rk64_atol, c.rk64_nsubsteps_guess, c.rk64_nsubsteps_min, c.rk64_nsubsteps_max};
-  P.adapt = {c.rtol, c.atol, c.first_step, c.max_steps, c.batched_lu_batch, c.batched_lu_wait};
+  P.adapt = {c.rtol, c.atol, c.first_step, c.max_steps, c.batched_lu_batch, c.batched_lu_wait,
+             c.batched_waves_per_simd};
   return P;
 }
 
@@ -458,6 +459,8 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
     return PP_CHEM_EINVAL;
   if (cfg->clean_init_massfrac != 0 && cfg->clean_init_massfrac != 1) return PP_CHEM_EINVAL;
   if (cfg->batched_box_direct != 0 && cfg->batched_box_direct != 1) return PP_CHEM_EINVAL;
+  if (cfg->batched_waves_per_simd < 0 || cfg->batched_waves_per_simd > 2) return PP_CHEM_EINVAL;
+  if (cfg->batched_waves_per_simd == 1 && cfg->batched_box_direct == 1) return PP_CHEM_EINVAL;  // no such build
   const int bs = cfg->block_size > 0 ? cfg->block_size : 64;
   if (bs > 64 || bs % 64 != 0) return PP_CHEM_EINVAL;  // kernels are built for 64-lane blocks
   pp_chem* h = new pp_chem();

@@ -95,7 +95,7 @@ hipError_t batched_1d_mode(const ReactParamsDev& P, double* y, const double* ysr
   } else
 #endif
   {
-    auto k = batched_kernel<M, LU_T, IO1D>;
+    auto k = P.adapt.lane_occ == 1 ? batched_kernel<M, LU_T, IO1D, 1> : batched_kernel<M, LU_T, IO1D, 2>;
     const int grid = batched_grid<M, MODE>(k, P, q);
     if (grid < 0) return grid == -2 ? hipErrorInvalidValue : hipErrorInvalidDevice;
     IO1D io{y, ysrc, rX, rXsrc};
@@ -119,7 +119,9 @@ hipError_t batched_box_mode(const ReactParamsDev& P, const BoxArgs& b, double dt
   } else
 #endif
   {
-    auto k = batched_kernel<M, LU_T, IOBox>;
+    // built at 2 waves per SIMD only: the 1-wave build of the table-driven
+    // drm19 box kernel returned failed cells, so 1-wave runs take the flat route
+    auto k = batched_kernel<M, LU_T, IOBox, 2>;
     const int grid = batched_grid<M, MODE>(k, P, q);
     if (grid < 0) return grid == -2 ? hipErrorInvalidValue : hipErrorInvalidDevice;
     IOBox io{b};

@@ -17,6 +17,7 @@ struct AdaptParamsDev {  // adaptive BDF tolerances
   double rtol, atol, first_step;
   int max_steps;
   int lu_batch, lu_wait;  // batched forms: wave-shared Jacobian/LU service (0: defaults 16, 4)
+  int lane_occ;           // batched forms: kernel build held to 1 or 2 waves per SIMD (0: 2)
 };
 
 struct ReactParamsDev {

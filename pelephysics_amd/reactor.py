@@ -66,6 +66,7 @@ class Config(C.Structure):
         ("max_steps", C.c_int), ("block_size", C.c_int), ("workspace_bytes", C.c_longlong),
         ("clean_init_massfrac", C.c_int), ("batched_waves_per_cu", C.c_int),
         ("batched_lu_batch", C.c_int), ("batched_lu_wait", C.c_int), ("batched_box_direct", C.c_int),
+        ("batched_waves_per_simd", C.c_int),
     ]
 
 
@@ -98,7 +99,7 @@ _KEYS = {
     "block_size": "block_size", "workspace_bytes": "workspace_bytes",
     "clean_init_massfrac": "clean_init_massfrac", "batched_waves_per_cu": "batched_waves_per_cu",
     "batched_lu_batch": "batched_lu_batch", "batched_lu_wait": "batched_lu_wait",
-    "batched_box_direct": "batched_box_direct",
+    "batched_box_direct": "batched_box_direct", "batched_waves_per_simd": "batched_waves_per_simd",
 }
 
 _lib = None
@@ -208,7 +209,8 @@ class ReactorBase:
             entry = lookup(mechanism, self.identifier, tuned)
             if entry:
                 mechanism = entry["kernels"]
-                for k in ("batched_waves_per_cu", "batched_lu_batch", "batched_lu_wait", "batched_box_direct"):
+                for k in ("batched_waves_per_cu", "batched_lu_batch", "batched_lu_wait", "batched_box_direct",
+                          "batched_waves_per_simd"):
                     if k in entry:
                         ode.setdefault(k, entry[k])
         cfg = Config()

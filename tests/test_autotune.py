@@ -6,12 +6,18 @@ from pelephysics_amd import autotune
 
 def test_candidates():
     c = autotune.candidates("drm19", "ReactorBDFAdaptiveMixed", "1d")
-    assert {(x["kernels"], x["batched_waves_per_cu"], x["batched_lu_batch"], x["batched_lu_wait"]) for x in c} == {
-        (k, w, b, t) for k in ("drm19", "drm19_table") for w in autotune.WAVES for b, t in autotune.LU_SERVICE}
-    assert len(autotune.candidates("drm19", "ReactorBDFAdaptiveMixed", "box")) == 2 * len(c)
+    got = {(x["kernels"], x["batched_waves_per_simd"], x["batched_waves_per_cu"], x["batched_lu_batch"],
+            x["batched_lu_wait"]) for x in c}
+    assert got == {(k, o, w, b, t) for k in ("drm19", "drm19_table") for o, w in autotune.OCCUPANCY
+                   for b, t in autotune.LU_SERVICE}
+    assert all(w <= 4 * o for _, o, w, _, _ in got)  # waves per CU within 4 SIMDs x waves per SIMD
+    box = autotune.candidates("drm19", "ReactorBDFAdaptiveMixed", "box")
+    one_wave = sum(1 for x in c if x["batched_waves_per_simd"] == 1)
+    assert len(box) == 2 * len(c) - one_wave  # no box-kernel route for the 1-wave build
+    assert not any(x["batched_waves_per_simd"] == 1 and x["batched_box_direct"] == 1 for x in box)
     c = autotune.candidates("dodecane_lu", "ReactorBDFAdaptive")
-    assert c == [{"kernels": "dodecane_lu", "batched_waves_per_cu": 0, "batched_lu_batch": 0,
-                  "batched_lu_wait": 0, "batched_box_direct": 0}]
+    assert c == [{"kernels": "dodecane_lu", "batched_waves_per_simd": 0, "batched_waves_per_cu": 0,
+                  "batched_lu_batch": 0, "batched_lu_wait": 0, "batched_box_direct": 0}]
 
 
 def test_lookup_reads_the_table(tmp_path, monkeypatch):

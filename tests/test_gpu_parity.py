@@ -428,3 +428,23 @@ def test_batched_box_routes_agree(models, ident):
         out.append([x.cpu().numpy() for x in (rY, T, rE, FC)])
     for a, b in zip(*out):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("mech", ["drm19", "drm19_table"])
+@pytest.mark.parametrize("ident", ["ReactorBDFAdaptiveMixed", "ReactorBDFBatchedFP64"])
+def test_batched_one_wave_build(models, oracles, mech, ident):
+    """The batched kernels built at 1 wave per SIMD (batched_waves_per_simd = 1,
+    1-D kernel, also for box calls) against the tight FP64 oracle."""
+    torch = _torch()
+    from pelephysics_amd.reactor import ReactorBase
+    m = models["drm19"]
+    st = W.subsample(W.drm19_grf(m, 64), np.arange(0, 64 ** 3, 1009))
+    ysrc, esrc = np.zeros((st.ncells, m.n_species)), np.zeros(st.ncells)
+    p = oracle.adapt_params(rtol=1e-13, atol=1e-20)
+    _, oy, _, _ = _oracle_1d(oracles["drm19"], "adaptive", st, ysrc, esrc, 1e-5, params=p)
+    ret, y, rX, R = _run_1d(torch, mech, ident, st, ysrc, esrc, 1e-5, rtol=1e-10, atol=1e-16,
+                            batched_waves_per_simd=1, batched_lu_batch=64, batched_lu_wait=8)
+    assert R.stats()["nfailed"] == 0
+    _compare_y(m.n_species + 1, y, oy)
+    with pytest.raises(Exception):
+        ReactorBase.create(ident).init(2, 4, mechanism=mech, batched_waves_per_simd=1, batched_box_direct=1)
