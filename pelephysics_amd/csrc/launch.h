@@ -16,11 +16,6 @@ namespace ppc {
 #define PPC_WAVE_BATCHED 0
 #endif
 
-template <class M, class = void>
-struct is_table_mech : std::false_type {};
-template <class M>
-struct is_table_mech<M, std::void_t<decltype(M::kTable)>> : std::true_type {};
-
 struct MechLauncher {
   const char* name;
   int ns, nr;

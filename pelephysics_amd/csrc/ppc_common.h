@@ -10,9 +10,9 @@
 #ifndef PPC_DEV
 #define PPC_DEV __device__
 #endif
-// Large per-cell routines (RHS, Jacobian, LU, T inversion): kept as real
-// calls so that each appears once in a kernel's code, whatever the number of
-// call sites in the integrator state machines.
+// Large per-cell routines (RHS, Jacobian, LU, T inversion).  Inlined: out of
+// line (-DPPC_HEAVY="__device__ __noinline__") the batched drm19 box kernel
+// measured 517 ms per 64^3 call against 431 ms inlined.
 #ifndef PPC_HEAVY
 #define PPC_HEAVY __device__ __forceinline__
 #endif
