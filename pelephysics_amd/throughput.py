@@ -52,6 +52,7 @@ def main(argv=None) -> dict:
     ap.add_argument("--dt", type=float, default=1e-5)
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--atol", type=float, default=1e-16)
+    ap.add_argument("--waves-per-simd", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--lu-batch", type=int, default=0)
     ap.add_argument("--lu-wait", type=int, default=0)
@@ -63,7 +64,7 @@ def main(argv=None) -> dict:
     dev = torch.device("cuda:0")
     R = ReactorBase.create(a.integrator)
     R.init(2, st.ncells, mechanism=a.mech, rtol=a.rtol, atol=a.atol, batched_waves_per_cu=a.waves_per_cu,
-           batched_lu_batch=a.lu_batch, batched_lu_wait=a.lu_wait)
+           batched_lu_batch=a.lu_batch, batched_lu_wait=a.lu_wait, batched_waves_per_simd=a.waves_per_simd)
     y0 = torch.from_numpy(st.y_1d()).to(dev)
     x0 = torch.from_numpy(st.rE.copy()).to(dev)
     y, x = y0.clone(), x0.clone()

@@ -135,3 +135,17 @@ def test_workspace_smaller_than_grid(models, ident):
                                    workspace_bytes=64 * rows * 8, **ode)
     assert R1.stats()["nfailed"] == 0
     assert np.array_equal(y_full, y_one) and np.array_equal(rX_full, rX_one)
+
+
+@pytest.mark.parametrize("mech", ["dodecane_lu", "heptane_lu_88sk"])
+def test_large_one_wave_build_tuned_service(models, oracles, mech):
+    """The batched FP32-LU kernel built at 1 wave per SIMD with the tuned
+    Jacobian/LU service (64 lanes or 8 trips) against the oracle."""
+    torch = _torch()
+    m, st, ysrc, esrc = _case(models, mech)
+    ret, y, rX, R = _run_1d(torch, mech, "ReactorBDFAdaptiveMixed", st, ysrc, esrc, 1e-5, rtol=1e-10,
+                            atol=1e-16, batched_waves_per_simd=1, batched_lu_batch=64, batched_lu_wait=8)
+    assert R.stats()["nfailed"] == 0
+    p = oracle.adapt_params(rtol=1e-13, atol=1e-20)
+    _, oy, orX, _ = _oracle_1d(oracles[mech], "adaptive", st, ysrc, esrc, 1e-5, params=p)
+    _compare_y(m.n_species + 1, y, oy)
