@@ -258,9 +258,10 @@ int finish(pp_chem* h, int ncells, hipStream_t s, float kernel_ms) {
 bool is_batched(int mode) { return mode == ppc::MODE_BATCHED_MIXED || mode == ppc::MODE_BATCHED_FP64; }
 
 // Per-lane workspace of mechanisms too large for private memory: one row of
-// doubles per lane and matrix element, for at most 8 waves per CU (the
-// resident waves of the 2-waves-per-SIMD kernels) and at most
-// cfg.workspace_bytes (default kWsMaxBytes).
+// doubles per lane and matrix element, for at most the resident waves per CU
+// of the kernel build (4 SIMDs x 2 waves for the 2-wave batched build, x 1
+// for the 1-wave batched build and the one-lane-per-cell kernels of the other
+// integrators) and at most cfg.workspace_bytes (default kWsMaxBytes).
 constexpr long long kWsMaxBytes = 32LL << 30;
 
 int prepare_workspace(pp_chem* h, int ncells) {
@@ -270,7 +271,8 @@ int prepare_workspace(pp_chem* h, int ncells) {
   if (rows == 0 || ncells == 0) return 0;
   int ncu = 0;
   HIPCHK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
-  long long blocks = std::min<long long>((ncells + 63) / 64, 8LL * std::max(ncu, 1));
+  const long long per_cu = (is_batched(h->cfg.mode) && h->cfg.batched_waves_per_simd != 1) ? 8 : 4;
+  long long blocks = std::min<long long>((ncells + 63) / 64, per_cu * std::max(ncu, 1));
   if (h->max_blocks > 0) blocks = std::min<long long>(blocks, h->max_blocks);
   const long long cap = h->cfg.workspace_bytes > 0 ? h->cfg.workspace_bytes : kWsMaxBytes;
   const long long lane_bytes = 64 * rows * (long long)sizeof(double);  // one 64-lane block
