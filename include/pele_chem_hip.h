@@ -80,7 +80,7 @@ typedef struct {
   int batched_waves_per_cu;
   /* Batched forms: a wave factors the Newton matrices of its waiting lanes
    * together once batched_lu_batch lanes (1..64) wait or a request has waited
-   * batched_lu_wait trips (0: 16 and 4); results do not depend on them. */
+   * batched_lu_wait trips (0: 64 and 32); results do not depend on them. */
   int batched_lu_batch, batched_lu_wait;
   /* Batched forms, box call: 0 (default) runs box -> flat, the 1-D kernel,
    * flat -> box (same per-cell operations, faster kernel build); 1 runs the

@@ -17,7 +17,8 @@ keeps the fastest:
   spills less;
 * batched_lu_batch / batched_lu_wait (batched forms only): how many waiting
   lanes, or how many trips of waiting, make a wave factor its lanes' Newton
-  matrices together (16 lanes or 4 trips, 64 lanes or 8 trips);
+  matrices together (16 lanes or 4 trips; 64 lanes or 8 trips; 64 lanes or
+  32 trips, i.e. nearly always the whole wave at once);
 * batched_box_direct (batched forms, box calls): the box kernel itself (1) or
   box -> flat state, the 1-D kernel, flat -> box (0).
 
@@ -44,7 +45,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 BATCHED = ("ReactorBDFAdaptiveMixed", "ReactorBDFBatchedFP64")
 OCCUPANCY = ((2, 8), (2, 6), (2, 4), (1, 4), (1, 3))  # (waves per SIMD, waves per CU)
-LU_SERVICE = ((16, 4), (64, 8))
+LU_SERVICE = ((16, 4), (64, 8), (64, 32))
 
 
 def arch() -> str:

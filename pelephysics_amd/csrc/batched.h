@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, OC
   bool need_jac = false, need_lu = false;
   int wait = 0;
   double yjac[n];
-  const int lu_batch = ap.lu_batch > 0 ? ap.lu_batch : 16, lu_wait = ap.lu_wait > 0 ? ap.lu_wait : 4;
+  const int lu_batch = ap.lu_batch > 0 ? ap.lu_batch : 64, lu_wait = ap.lu_wait > 0 ? ap.lu_wait : 32;
 
   int cell = 0, cid = 0;
   bool active = false;

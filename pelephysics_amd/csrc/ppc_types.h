@@ -16,7 +16,7 @@ struct RK64ParamsDev {  // ReactorRK64 ode.* (Source/Reactions/ReactorRK64.H:131
 struct AdaptParamsDev {  // adaptive BDF tolerances
   double rtol, atol, first_step;
   int max_steps;
-  int lu_batch, lu_wait;  // batched forms: wave-shared Jacobian/LU service (0: defaults 16, 4)
+  int lu_batch, lu_wait;  // batched forms: wave-shared Jacobian/LU service (0: defaults 64, 32)
   int lane_occ;           // batched forms: kernel build held to 1 or 2 waves per SIMD (0: 2)
 };
 

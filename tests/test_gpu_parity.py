@@ -448,3 +448,28 @@ def test_batched_one_wave_build(models, oracles, mech, ident):
     _compare_y(m.n_species + 1, y, oy)
     with pytest.raises(Exception):
         ReactorBase.create(ident).init(2, 4, mechanism=mech, batched_waves_per_simd=1, batched_box_direct=1)
+
+
+@pytest.mark.parametrize("ident", ["ReactorBDFAdaptiveMixed", "ReactorBDFBatchedFP64"])
+def test_lu_service_parameters_bitwise(models, ident):
+    """The wave-shared Jacobian/LU service only decides when a wave factors
+    its waiting lanes' matrices: every request served at once (lu_batch 1),
+    (16 lanes, 4 trips), (64, 8) and the default (64, 32) give the same
+    states, step counts and per-cell work, bit for bit, on 2-wave and 1-wave
+    builds."""
+    torch = _torch()
+    m = models["drm19"]
+    st = W.subsample(W.drm19_grf(m, 64), np.arange(0, 64 ** 3, 97))
+    ysrc, esrc = np.zeros((st.ncells, m.n_species)), np.zeros(st.ncells)
+    runs = []
+    for occ in (2, 1):
+        for b, t in ((1, 1), (16, 4), (64, 8), (0, 0)):
+            _, y, rX, R = _run_1d(torch, "drm19", ident, st, ysrc, esrc, 1e-5, rtol=1e-10, atol=1e-16,
+                                  batched_waves_per_simd=occ, batched_lu_batch=b, batched_lu_wait=t)
+            s = R.stats()
+            assert s["nfailed"] == 0
+            runs.append((y, rX, R.cell_steps(st.ncells), [s[k] for k in ("nsteps", "nrhs", "njac", "nlu")]))
+    y0, rX0, fc0, w0 = runs[0]
+    for y, rX, fc, w in runs[1:]:
+        assert np.array_equal(y, y0) and np.array_equal(rX, rX0)
+        assert np.array_equal(fc, fc0) and w == w0
