@@ -10,7 +10,8 @@ the 1-D form on device pointers; without --chained every call restarts from
 the initial state, with --chained the calls continue from each other (as a
 flow solver's successive chemistry steps do).  Prints one JSON line: cells/s
 from the kernel time of each call (HIP events around the launch), the
-per-cell work counters and the matrix workspace per resident cell.
+per-cell work counters (means, and the largest and 99th-percentile step
+count of a cell) and the matrix workspace per resident cell.
 """
 from __future__ import annotations
 
@@ -91,9 +92,12 @@ def main(argv=None) -> dict:
         R.react_1d(y, ys, x, xs, a.dt)
         s = R.stats()
         kms.append(s["kernel_ms"])
-        per_call.append({k: s[k] / max(1, s["ncells"]) for k in ("nsteps", "nrhs", "njac", "nlu")})
+        steps = R.cell_steps(st.ncells)
+        per_call.append(dict({k: s[k] / max(1, s["ncells"]) for k in ("nsteps", "nrhs", "njac", "nlu")},
+                             max_steps=int(steps.max()), p99_steps=float(np.percentile(steps, 99))))
         print(f"[throughput] call {call + 1}/{a.calls}: kernel {s['kernel_ms']:.1f} ms, "
-              f"{s['nsteps'] / max(1, s['ncells']):.1f} steps/cell, failed {s['nfailed']}",
+              f"{s['nsteps'] / max(1, s['ncells']):.1f} steps/cell (max {per_call[-1]['max_steps']}), "
+              f"failed {s['nfailed']}",
               file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
