@@ -149,3 +149,21 @@ def test_large_one_wave_build_tuned_service(models, oracles, mech):
     p = oracle.adapt_params(rtol=1e-13, atol=1e-20)
     _, oy, orX, _ = _oracle_1d(oracles[mech], "adaptive", st, ysrc, esrc, 1e-5, params=p)
     _compare_y(m.n_species + 1, y, oy)
+
+
+def test_large_lu_service_parameters_bitwise(models):
+    """With the matrices in the workspace too, the Jacobian/LU service
+    parameters (every request served at once; 16 lanes or 4 trips; the default
+    64 lanes or 32 trips) leave states and per-cell step counts bitwise
+    unchanged."""
+    torch = _torch()
+    mech = "dodecane_lu"
+    m, st, ysrc, esrc = _case(models, mech, ncells=128)
+    runs = []
+    for b, t in ((1, 1), (16, 4), (0, 0)):
+        _, y, rX, R = _run_1d(torch, mech, "ReactorBDFAdaptiveMixed", st, ysrc, esrc, 1e-5, rtol=1e-10,
+                              atol=1e-16, batched_lu_batch=b, batched_lu_wait=t)
+        assert R.stats()["nfailed"] == 0
+        runs.append((y, rX, R.cell_steps(st.ncells)))
+    for y, rX, fc in runs[1:]:
+        assert np.array_equal(y, runs[0][0]) and np.array_equal(rX, runs[0][1]) and np.array_equal(fc, runs[0][2])
