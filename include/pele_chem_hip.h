@@ -120,7 +120,12 @@ int pp_chem_list_mechanisms(char* buf, int buflen);
 
 /* Box form: device-resident Array4 views over [lo, hi] (inclusive), updated
  * in place (rY, T, rE, FC); rYsrc, rEsrc read; mask ignored as in the
- * reference (ReactorBDF.cpp:322).  FC may have p == NULL. */
+ * reference (ReactorBDF.cpp:322).  FC may have p == NULL.
+ * The batched modes take the cells from a work queue; when the previous call
+ * on this handle had the same cell count and the same rY data pointer (the
+ * same cells, e.g. the next chemistry step of the same box), the queue visits
+ * them in the order of that call's per-cell cost, longest first.  Results do
+ * not depend on the order. */
 int pp_chem_react_box(pp_chem* h, const int lo[3], const int hi[3], const pp_array4* rY,
                       const pp_array4* rYsrc, const pp_array4* T, const pp_array4* rE,
                       const pp_array4* rEsrc, const pp_array4* FC, const pp_array4* mask,
@@ -128,7 +133,9 @@ int pp_chem_react_box(pp_chem* h, const int lo[3], const int hi[3], const pp_arr
 
 /* 1-D form: y[icell*(N+1)+k] (k = N is T), ysrc[icell*N+k], rX/rXsrc[icell];
  * on_device = 0: host pointers (copied in and out, as the reference does on
- * GPU builds, ReactorBDF.cpp:155-171,295-302); 1: device pointers. */
+ * GPU builds, ReactorBDF.cpp:155-171,295-302); 1: device pointers.  The cost
+ * order of the batched modes applies when the previous call had the same
+ * ncells and the same y pointer (host or device): reuse the buffers. */
 int pp_chem_react_1d(pp_chem* h, double* y, double* ysrc, double* rX, double* rXsrc,
                      double* dt_react, double* time, int ncells, int on_device, void* stream);
 
