@@ -62,6 +62,7 @@ public:
     pp.query("batched_lu_batch", m_cfg.batched_lu_batch);
     pp.query("batched_lu_wait", m_cfg.batched_lu_wait);
     pp.query("batched_waves_per_simd", m_cfg.batched_waves_per_simd);
+    pp.query("batched_cells_per_wave", m_cfg.batched_cells_per_wave);
     pp.query("batched_box_direct", m_cfg.batched_box_direct);
     pp.query("workspace_bytes", m_cfg.workspace_bytes);
     bool clean = false;

@@ -90,6 +90,13 @@ typedef struct {
    * default) or to 1 (more registers per lane, fewer spills; 1-D kernel only,
    * so not with batched_box_direct = 1). */
   int batched_waves_per_simd;
+  /* Batched forms, cells per 64-lane wavefront: 0 (default) integrates one
+   * cell per lane (64 per wave); 8, 4, 2 or 1 give each cell a group of 8, 16,
+   * 32 or 64 lanes that share its equations, reactions and LU rows, so its
+   * state stays in registers (table-driven mechanism kernels only; the values
+   * a mechanism supports depend on its size; box calls go through flat
+   * state). */
+  int batched_cells_per_wave;
 } pp_chem_config;
 
 /* amrex::Array4<Real> layout: element (i,j,k,n) is at
@@ -163,10 +170,30 @@ int pp_chem_unflatten(pp_chem* h, const int lo[3], const int hi[3], const pp_arr
  * device work (queue reset, cost ordering, integration kernel) is captured
  * once as a HIP graph and replayed; the graph is rebuilt when an array,
  * ncells or dt changes.  Statistics and per-cell counters are those of the
- * last call; kernel_ms covers all calls.  No counterpart in the reference,
+ * last call; kernel_ms covers all calls.  Returns PP_CHEM_EINTEGRATE when a
+ * cell failed in any of the calls (its partly integrated state is what the
+ * next call starts from).  No counterpart in the reference,
  * whose callers loop over ReactorBDF::react (ReactEval/main.cpp:475-686). */
 int pp_chem_react_1d_chained(pp_chem* h, double* y, double* ysrc, double* rX, double* rXsrc,
                              double* dt_react, double* time, int ncells, int ncalls, void* stream);
+
+/* The mechanism functions the integrators call, over a batch of cells on the
+ * device (one cell per lane), in the reference's SI units:
+ * pp_chem_production_rate replaces productionRate(wdot, sc, T)
+ * (Mechanisms/drm19/mechanism.H:4421-5667): sc[c*N + k] in mol/m^3, T[c] in K,
+ * wdot[c*N + k] in mol/m^3/s;
+ * pp_chem_ajacobian replaces aJacobian(J, sc, T, consP)
+ * (Mechanisms/drm19/mechanism.H:10307-15357): J[c*(N+1)^2 + k*(N+1) + m] =
+ * d wdot_m / d c_k, column N = d / dT, row N = the temperature equation.
+ * pp_chem_progress_rate gives the net rate of progress q_f - q_r of every
+ * reaction (the rates comp_qfqr forms, Mechanisms/drm19/mechanism.H:3600,
+ * and progressRateFR returns split, mechanism.cpp:120): q[c*NR + j].
+ * Synchronous on `stream`; the handle selects the mechanism. */
+int pp_chem_production_rate(pp_chem* h, const double* sc, const double* T, int ncells, double* wdot,
+                            void* stream);
+int pp_chem_ajacobian(pp_chem* h, const double* sc, const double* T, int ncells, int consP, double* J,
+                      void* stream);
+int pp_chem_progress_rate(pp_chem* h, const double* sc, const double* T, int ncells, double* q, void* stream);
 
 int pp_chem_get_stats(const pp_chem* h, pp_chem_stats* out);
 int pp_chem_num_reactions(const pp_chem* h);

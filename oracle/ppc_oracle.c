@@ -971,7 +971,7 @@ static int adaptive_core(sys_t* s, const ppo_adapt_params* p, double* y0, double
     int current_jac = 0, accepted = 0, n_iter = 0;
     double t_new = t, error_norm = 0.0, safety = 0.9;
     while (!accepted) {
-      if (h_abs < min_step) { status = -3; goto done; }
+      if (!(h_abs >= min_step)) { status = -3; goto done; } /* NaN step sizes end the cell too */
       double h = h_abs;
       t_new = t + h;
       if (t_new - t_bound > 0) {

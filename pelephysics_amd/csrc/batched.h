@@ -126,6 +126,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, OC
   int wait = 0;
   double yjac[n];
   const int lu_batch = ap.lu_batch > 0 ? ap.lu_batch : 64, lu_wait = ap.lu_wait > 0 ? ap.lu_wait : 32;
+  // RHS budget per cell: every path either accepts a step (at most max_steps)
+  // or shrinks h until the minimum-step test ends the cell; the budget bounds
+  // the lane's work whatever the arithmetic does
+  const long long rhs_cap = rhs_budget(ap.max_steps);
 
   int cell = 0, cid = 0;
   bool active = false;
@@ -209,7 +213,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, OC
   // top of one step attempt: predictor, weights, LU if needed
   auto begin_attempt = [&]() -> bool {
     const double min_step = 10 * fabs(nextafter(t, INFINITY) - t);
-    if (h_abs < min_step) {
+    if (!(h_abs >= min_step)) {  // also ends a cell whose step size is NaN
       status = -3;
       return false;
     }
@@ -278,6 +282,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, OC
     }
     if (need_lu) {  // not served yet: no RHS, no Newton iteration this trip
       ++wait;
+      continue;
+    }
+    if (nrhs >= rhs_cap) {  // out of budget: the cell fails with the step-limit status
+      status = -2;
+      finish_cell();
       continue;
     }
     // ---- the one RHS evaluation of this trip

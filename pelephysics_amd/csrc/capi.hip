@@ -76,6 +76,13 @@ __global__ void __launch_bounds__(256) reduce_stats(const int* __restrict__ nste
   }
 }
 
+// Counts the cells of the last launch that failed (negative step count) into
+// *nfail, which the caller clears once per chain of calls.
+__global__ void __launch_bounds__(256) count_failed(const int* __restrict__ nsteps, int ncells, int* __restrict__ nfail) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x)
+    if (nsteps[c] < 0) atomicAdd(nfail, 1);
+}
+
 __global__ void init_stats(unsigned long long* out) {
   if (threadIdx.x < 9) out[threadIdx.x] = (threadIdx.x == 8) ? 0x7fffffffULL : 0ULL;
 }
@@ -181,6 +188,8 @@ struct pp_chem {
   GraphKey gkey{};
   // persistent-grid bound of the batched forms: batched_waves_per_cu x CUs
   int max_blocks = 0;
+  // failed cells over every call of a chain (pp_chem_react_1d_chained)
+  int* d_fail = nullptr;
 };
 
 namespace {
@@ -267,12 +276,17 @@ constexpr long long kWsMaxBytes = 32LL << 30;
 int prepare_workspace(pp_chem* h, int ncells) {
   h->P.ws = nullptr;
   h->P.ws_lanes = 0;
-  const long long rows = h->L->ws_rows(h->cfg.mode);
+  const bool group = is_batched(h->cfg.mode) && h->cfg.batched_cells_per_wave != 0;
+  const long long rows = group ? h->L->group_ws_rows(h->cfg.batched_cells_per_wave) : h->L->ws_rows(h->cfg.mode);
   if (rows == 0 || ncells == 0) return 0;
   int ncu = 0;
   HIPCHK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
-  const long long per_cu = (is_batched(h->cfg.mode) && h->cfg.batched_waves_per_simd != 1) ? 8 : 4;
-  long long blocks = std::min<long long>((ncells + 63) / 64, per_cu * std::max(ncu, 1));
+  const long long per_cu =
+      (is_batched(h->cfg.mode) && (group || h->cfg.batched_waves_per_simd != 1)) ? 8 : 4;
+  // blocks of 64 lanes that can hold a cell each: one cell per lane, or per
+  // group of 64 / batched_cells_per_wave lanes
+  const long long lanes_per_cell = group ? 64 / h->cfg.batched_cells_per_wave : 1;
+  long long blocks = std::min<long long>(((long long)ncells * lanes_per_cell + 63) / 64, per_cu * std::max(ncu, 1));
   if (h->max_blocks > 0) blocks = std::min<long long>(blocks, h->max_blocks);
   const long long cap = h->cfg.workspace_bytes > 0 ? h->cfg.workspace_bytes : kWsMaxBytes;
   const long long lane_bytes = 64 * rows * (long long)sizeof(double);  // one 64-lane block
@@ -313,14 +327,21 @@ ppc::QueueArgs prepare_queue(pp_chem* h, int ncells, const void* key, hipStream_
 }
 
 // One 1-D call's device work on stream s (the queue in cost order when the
-// previous call integrated the same cells).
+// previous call integrated the same cells).  With count_fail, the cells that
+// failed in this call are added to h->d_fail.
 hipError_t enqueue_1d(pp_chem* h, double* y, double* ysrc, double* rX, double* rXsrc, double dt, int ncells,
-                      const void* key, hipStream_t s) {
+                      const void* key, hipStream_t s, bool count_fail = false) {
+  hipError_t e;
   if (is_batched(h->cfg.mode)) {
     const ppc::QueueArgs q = prepare_queue(h, ncells, key, s);
-    return h->L->batched_1d(h->cfg.mode, h->P, y, ysrc, rX, rXsrc, dt, q, counters(h), s);
+    e = h->L->batched_1d(h->cfg.mode, h->P, y, ysrc, rX, rXsrc, dt, q, counters(h), s);
+  } else {
+    e = h->L->react_1d(h->cfg.mode, h->P, y, ysrc, rX, rXsrc, dt, ncells, counters(h), h->cfg.block_size, s);
   }
-  return h->L->react_1d(h->cfg.mode, h->P, y, ysrc, rX, rXsrc, dt, ncells, counters(h), h->cfg.block_size, s);
+  if (e != hipSuccess || !count_fail || ncells == 0) return e;
+  const int blocks = std::min(1024, (ncells + 255) / 256);
+  hipLaunchKernelGGL(count_failed, dim3(blocks), dim3(256), 0, s, h->d_cnt, ncells, h->d_fail);
+  return hipGetLastError();
 }
 
 void drop_graph(pp_chem* h) {
@@ -337,7 +358,7 @@ ppc::ReactParamsDev to_dev(const pp_chem_config& c) {
            c.bdf_gmres_precond, c.bdf_scheme, c.bdf_nonlinear_tol, c.bdf_gmres_tol};
   P.rk64 = {c.rk64_atol, c.rk64_nsubsteps_guess, c.rk64_nsubsteps_min, c.rk64_nsubsteps_max};
   P.adapt = {c.rtol, c.atol, c.first_step, c.max_steps, c.batched_lu_batch, c.batched_lu_wait,
-             c.batched_waves_per_simd};
+             c.batched_waves_per_simd, c.batched_cells_per_wave};
   return P;
 }
 
@@ -463,6 +484,11 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
   if (cfg->batched_box_direct != 0 && cfg->batched_box_direct != 1) return PP_CHEM_EINVAL;
   if (cfg->batched_waves_per_simd < 0 || cfg->batched_waves_per_simd > 2) return PP_CHEM_EINVAL;
   if (cfg->batched_waves_per_simd == 1 && cfg->batched_box_direct == 1) return PP_CHEM_EINVAL;  // no such build
+  if (cfg->batched_cells_per_wave != 0) {
+    // a group kernel of this size must exist for the mechanism; box calls go through flat state
+    if (!is_batched(cfg->mode) || cfg->batched_box_direct == 1) return PP_CHEM_EINVAL;
+    if (L->group_ws_rows(cfg->batched_cells_per_wave) <= 0) return PP_CHEM_EINVAL;
+  }
   const int bs = cfg->block_size > 0 ? cfg->block_size : 64;
   if (bs > 64 || bs % 64 != 0) return PP_CHEM_EINVAL;  // kernels are built for 64-lane blocks
   pp_chem* h = new pp_chem();
@@ -480,6 +506,7 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
   }
   if (e == hipSuccess) e = hipMalloc(&h->d_red, sizeof(unsigned long long) * 9);
   if (e == hipSuccess) e = hipMalloc(&h->d_queue, sizeof(int) * (1 + 2 * NBINS));
+  if (e == hipSuccess) e = hipMalloc(&h->d_fail, sizeof(int));
   if (e == hipSuccess) e = hipEventCreate(&h->ev0);
   if (e == hipSuccess) e = hipEventCreate(&h->ev1);
   if (e != hipSuccess) {
@@ -501,6 +528,7 @@ void pp_chem_destroy(pp_chem* h) {
   if (h->d_cnt) (void)hipFree(h->d_cnt);
   if (h->d_red) (void)hipFree(h->d_red);
   if (h->d_queue) (void)hipFree(h->d_queue);
+  if (h->d_fail) (void)hipFree(h->d_fail);
   if (h->d_order) (void)hipFree(h->d_order);
   if (h->d_stage) (void)hipFree(h->d_stage);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -510,7 +538,12 @@ void pp_chem_destroy(pp_chem* h) {
 
 int pp_chem_num_species(const pp_chem* h) { return h ? h->L->ns : PP_CHEM_EINVAL; }
 int pp_chem_num_reactions(const pp_chem* h) { return h ? h->L->nr : PP_CHEM_EINVAL; }
-long long pp_chem_workspace_rows(const pp_chem* h) { return h ? h->L->ws_rows(h->cfg.mode) : PP_CHEM_EINVAL; }
+long long pp_chem_workspace_rows(const pp_chem* h) {
+  if (!h) return PP_CHEM_EINVAL;
+  if (is_batched(h->cfg.mode) && h->cfg.batched_cells_per_wave != 0)
+    return h->L->group_ws_rows(h->cfg.batched_cells_per_wave);
+  return h->L->ws_rows(h->cfg.mode);
+}
 
 int pp_chem_adjoint_1d(pp_chem* h, double* y, const double* rX, int ncells, int nmacro, double dt,
                        int spec_a, int spec_b, double* out, double* grad, void* stream) {
@@ -726,11 +759,15 @@ int pp_chem_react_1d_chained(pp_chem* h, double* y, double* ysrc, double* rX, do
   }
   const double dt = *dt_react;
   HIPCHK(h, hipEventRecord(h->ev0, s));
+  // failures are counted over every call of the chain: a cell that fails in
+  // an earlier call fails the chain even if a later call integrates it
+  hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, h->d_fail, 1);
+  HIPCHK(h, hipGetLastError());
   int done = 0;
   if (ncells > 0 && is_batched(h->cfg.mode) &&
       !(h->hist_valid && h->hist_ncells == ncells && h->hist_key == y)) {
     // the first call sets the cost history the captured calls order by
-    const hipError_t e = enqueue_1d(h, y, ysrc, rX, rXsrc, dt, ncells, y, s);
+    const hipError_t e = enqueue_1d(h, y, ysrc, rX, rXsrc, dt, ncells, y, s, true);
     if (e != hipSuccess) return hip_fail(h, e, "react_1d launch");
     done = 1;
   }
@@ -739,7 +776,7 @@ int pp_chem_react_1d_chained(pp_chem* h, double* y, double* ysrc, double* rX, do
     if (!h->gexec || !(key == h->gkey)) {
       drop_graph(h);
       HIPCHK(h, hipStreamBeginCapture(h->gstream, hipStreamCaptureModeThreadLocal));
-      const hipError_t e = enqueue_1d(h, y, ysrc, rX, rXsrc, dt, ncells, y, h->gstream);
+      const hipError_t e = enqueue_1d(h, y, ysrc, rX, rXsrc, dt, ncells, y, h->gstream, true);
       hipGraph_t g = nullptr;
       const hipError_t e2 = hipStreamEndCapture(h->gstream, &g);
       if (e != hipSuccess) {
@@ -761,9 +798,47 @@ int pp_chem_react_1d_chained(pp_chem* h, double* y, double* ysrc, double* rX, do
   HIPCHK(h, hipEventSynchronize(h->ev1));
   float ms = 0.f;
   HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  int nfail = 0;
+  HIPCHK(h, hipMemcpy(&nfail, h->d_fail, sizeof(int), hipMemcpyDeviceToHost));
   const int r = finish(h, ncells, s, ms);
   if (h->cfg.mod_reactor) *time += ncalls * dt;
+  if (r >= 0 && nfail > 0)
+    return fail(h, PP_CHEM_EINTEGRATE,
+                std::to_string(nfail) + " cell integration(s) failed over the chained calls (NaN, step limit or "
+                "step too small)");
   return r;
+}
+
+int pp_chem_production_rate(pp_chem* h, const double* sc, const double* T, int ncells, double* wdot, void* stream) {
+  if (!h) return PP_CHEM_EINVAL;
+  if (ncells < 0 || (ncells > 0 && (!sc || !T || !wdot))) return fail(h, PP_CHEM_EINVAL, "bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  const hipError_t e = h->L->mech_eval(0, sc, T, ncells, wdot, s);
+  if (e != hipSuccess) return hip_fail(h, e, "productionRate launch");
+  HIPCHK(h, hipStreamSynchronize(s));
+  return PP_CHEM_OK;
+}
+
+int pp_chem_progress_rate(pp_chem* h, const double* sc, const double* T, int ncells, double* q, void* stream) {
+  if (!h) return PP_CHEM_EINVAL;
+  if (ncells < 0 || (ncells > 0 && (!sc || !T || !q))) return fail(h, PP_CHEM_EINVAL, "bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  const hipError_t e = h->L->mech_eval(3, sc, T, ncells, q, s);
+  if (e != hipSuccess) return hip_fail(h, e, "progress rate launch");
+  HIPCHK(h, hipStreamSynchronize(s));
+  return PP_CHEM_OK;
+}
+
+int pp_chem_ajacobian(pp_chem* h, const double* sc, const double* T, int ncells, int consP, double* J,
+                      void* stream) {
+  if (!h) return PP_CHEM_EINVAL;
+  if (ncells < 0 || (consP != 0 && consP != 1) || (ncells > 0 && (!sc || !T || !J)))
+    return fail(h, PP_CHEM_EINVAL, "bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  const hipError_t e = h->L->mech_eval(1 + consP, sc, T, ncells, J, s);
+  if (e != hipSuccess) return hip_fail(h, e, "aJacobian launch");
+  HIPCHK(h, hipStreamSynchronize(s));
+  return PP_CHEM_OK;
 }
 
 int pp_chem_get_stats(const pp_chem* h, pp_chem_stats* out) {

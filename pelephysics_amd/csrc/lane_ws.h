@@ -75,4 +75,39 @@ constexpr long long integrator_ws_rows(int mode) {
   }
 }
 
+// ---- batched group kernel (group_bdf.h): G lanes per cell
+template <class M, class = void>
+struct has_table : std::false_type {};
+template <class M>
+struct has_table<M, std::void_t<decltype(M::kTable)>> : std::integral_constant<bool, M::kTable> {};
+
+// Groups of G lanes per cell exist for the table-driven mechanisms with at most
+// three equations per lane (NEQ <= 3 G) and at least half the lanes holding
+// an equation (G <= 2 NEQ).
+template <class M, int G>
+constexpr bool group_ok() {
+  return has_table<M>::value && M::NEQ <= 3 * G && G <= 2 * M::NEQ;
+}
+
+// Device workspace of one group: the n x n FP64 Jacobian accumulator and its
+// rows (at most FP64)
+template <int n, int G>
+constexpr long long group_ws_doubles_max() {
+  return (long long)n * n + (long long)((n + G - 1) / G) * n * G;
+}
+
+// Workspace rows per lane (doubles, 64-lane blocks as lane_ws) of the group
+// kernel with cells_per_wave cells per wave; 0 when there is no such kernel.
+template <class M>
+long long group_ws_rows(int cells_per_wave) {
+  constexpr int n = M::NEQ;
+  switch (cells_per_wave) {
+    case 8: return group_ok<M, 8>() ? (8 * group_ws_doubles_max<n, 8>() + 63) / 64 : 0;
+    case 4: return group_ok<M, 16>() ? (4 * group_ws_doubles_max<n, 16>() + 63) / 64 : 0;
+    case 2: return group_ok<M, 32>() ? (2 * group_ws_doubles_max<n, 32>() + 63) / 64 : 0;
+    case 1: return group_ok<M, 64>() ? (group_ws_doubles_max<n, 64>() + 63) / 64 : 0;
+    default: return 0;
+  }
+}
+
 }  // namespace ppc

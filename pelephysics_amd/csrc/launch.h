@@ -34,9 +34,19 @@ struct MechLauncher {
   hipError_t (*adjoint_1d)(const ReactParamsDev& P, const AdjointArgs& a, hipStream_t s);
   // workspace rows per lane a mode needs (lane_ws.h); 0: none
   long long (*ws_rows)(int mode);
+  // workspace rows per lane of the batched group kernel with cells_per_wave
+  // cells per wave (group_bdf.h); <= 0: no such kernel for this mechanism
+  long long (*group_ws_rows)(int cells_per_wave);
   // box <-> flat state (flatten.h): unflatten = 0 flattens, 1 unflattens
   hipError_t (*flatten)(const FlatArgs& a, int unflatten, hipStream_t s);
+  // the mechanism functions over a batch of cells: what = 0 productionRate,
+  // 1 / 2 aJacobian with consP = 0 / 1, 3 net rates of progress (SI units, as
+  // the reference's)
+  hipError_t (*mech_eval)(int what, const double* sc, const double* T, int ncells, double* out, hipStream_t s);
 };
+
+template <class M>
+hipError_t launch_mech_eval(int what, const double* sc, const double* T, int ncells, double* out, hipStream_t s);
 
 template <class M>
 hipError_t launch_flatten(const FlatArgs& a, int unflatten, hipStream_t s);
@@ -107,7 +117,8 @@ hipError_t launch_box(int mode, const ReactParamsDev& P, const BoxArgs& b, doubl
                                    &launch_1d<TYPE>, &launch_box<TYPE>,                        \
                                    &launch_batched_1d<TYPE>, &launch_batched_box<TYPE>,        \
                                    &launch_adjoint<TYPE>, &integrator_ws_rows<TYPE>,           \
-                                   &launch_flatten<TYPE>};                                     \
+                                   &group_ws_rows<TYPE>,                                       \
+                                   &launch_flatten<TYPE>, &launch_mech_eval<TYPE>};            \
     return &L;                                                                                 \
   }                                                                                            \
   }

@@ -6,6 +6,7 @@
 #include "batched.h"
 #include "wave_bdf.h"
 #include "wave_adjoint.h"
+#include "group_bdf.h"
 #include "flatten.h"
 
 namespace ppc {
@@ -80,10 +81,56 @@ int batched_grid(K kernel, const ReactParamsDev& P, const QueueArgs& q) {
   return grid;
 }
 
+// The group kernel (group_bdf.h) with G lanes per cell: a persistent grid of
+// every resident block, at most q.max_blocks and at most the blocks the
+// workspace holds (group_ws_rows rows of 64 lanes per block).
+template <class T, typename LU_T, int G>
+hipError_t launch_group(const ReactParamsDev& P, double* y, const double* ysrc, double* rX, const double* rXsrc,
+                        double dt, QueueArgs q, CellStats st, hipStream_t s) {
+  static_assert(group_ws_doubles<T::NEQ, G, LU_T>() <= group_ws_doubles_max<T::NEQ, G>(), "workspace layout");
+  // registers of the LU rows: above ~100 VGPRs hold the build to one wave per SIMD
+  constexpr int lu_vgprs = (int)(((T::NEQ + G - 1) / G) * T::NEQ * sizeof(LU_T) / 4);
+  auto k = group_bdf_kernel<T, LU_T, G, (lu_vgprs > 100 ? 1 : 2)>;
+  int grid = resident_blocks(k, (long long)q.ncells * G);
+  if (grid < 0) return hipErrorInvalidDevice;
+  if (q.max_blocks > 0 && q.max_blocks < grid) grid = q.max_blocks;
+  if (!P.ws || P.ws_lanes < 64 || P.ws_lanes % 64 != 0) return hipErrorInvalidValue;
+  if (P.ws_lanes / 64 < grid) grid = (int)(P.ws_lanes / 64);
+  IO1D io{y, ysrc, rX, rXsrc};
+  hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
+  return hipGetLastError();
+}
+
+template <class M, typename LU_T>
+hipError_t batched_group(const ReactParamsDev& P, double* y, const double* ysrc, double* rX, const double* rXsrc,
+                         double dt, QueueArgs q, CellStats st, hipStream_t s) {
+  using Tab = typename wave_tables<M>::type;
+  if constexpr (!std::is_void<Tab>::value) {
+    switch (P.adapt.group) {
+      case 8:
+        if constexpr (group_ok<M, 8>()) return launch_group<Tab, LU_T, 8>(P, y, ysrc, rX, rXsrc, dt, q, st, s);
+        break;
+      case 4:
+        if constexpr (group_ok<M, 16>()) return launch_group<Tab, LU_T, 16>(P, y, ysrc, rX, rXsrc, dt, q, st, s);
+        break;
+      case 2:
+        if constexpr (group_ok<M, 32>()) return launch_group<Tab, LU_T, 32>(P, y, ysrc, rX, rXsrc, dt, q, st, s);
+        break;
+      case 1:
+        if constexpr (group_ok<M, 64>()) return launch_group<Tab, LU_T, 64>(P, y, ysrc, rX, rXsrc, dt, q, st, s);
+        break;
+      default:
+        break;
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
 template <class M, int MODE>
 hipError_t batched_1d_mode(const ReactParamsDev& P, double* y, const double* ysrc, double* rX, const double* rXsrc,
                            double dt, QueueArgs q, CellStats st, hipStream_t s) {
   using LU_T = typename std::conditional<MODE == MODE_BATCHED_MIXED, float, double>::type;
+  if (P.adapt.group > 0) return batched_group<M, LU_T>(P, y, ysrc, rX, rXsrc, dt, q, st, s);
 #if PPC_WAVE_BATCHED
   using Tab = typename wave_tables<M>::type;
   if constexpr (!std::is_void<Tab>::value) {
@@ -157,11 +204,50 @@ hipError_t launch_flatten(const FlatArgs& a, int unflatten, hipStream_t s) {
   return hipGetLastError();
 }
 
+// productionRate (Mechanisms/drm19/mechanism.H:4421-5667), aJacobian
+// (:10307-15357) or the net rates of progress q_f - q_r (comp_qfqr, :3600) of
+// one cell per lane: sc[c*NS + k] (mol/m^3), T[c] (K); out[c*NS + k] = wdot_k
+// (mol/m^3/s), out[c*n*n + ...] = the column-major Jacobian of cell c, or
+// out[c*NR + j] = q_j (mol/m^3/s), written in place in the output array.
+template <class M>
+__global__ void __launch_bounds__(64) mech_eval_kernel(int what, const double* __restrict__ sc,
+                                                       const double* __restrict__ T, int ncells, double* out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  double s[M::NS];
+  for (int k = 0; k < M::NS; ++k) s[k] = sc[(long long)c * M::NS + k];
+  const double Tv = T[c];
+  if (what == 0) {
+    double w[M::NS];
+    M::production_rate(w, s, Tv);
+    for (int k = 0; k < M::NS; ++k) out[(long long)c * M::NS + k] = w[k];
+  } else if (what == 3) {
+    double* q = out + (long long)c * M::NR;  // written in place: NR can be large
+    M::progress_rates(q, s, Tv);
+  } else {
+    double* J = out + (long long)c * M::NEQ * M::NEQ;
+    if (what == 1)
+      M::template jacobian<0>(J, s, Tv);
+    else
+      M::template jacobian<1>(J, s, Tv);
+  }
+}
+
+template <class M>
+hipError_t launch_mech_eval(int what, const double* sc, const double* T, int ncells, double* out, hipStream_t s) {
+  if (what < 0 || what > 3) return hipErrorInvalidValue;
+  if (ncells <= 0) return hipSuccess;
+  hipLaunchKernelGGL(mech_eval_kernel<M>, dim3((ncells + 63) / 64), dim3(64), 0, s, what, sc, T, ncells, out);
+  return hipGetLastError();
+}
+
 }  // namespace ppc
 
 #define PPC_INSTANTIATE_FLATTEN(TYPE, MODE)                                                        \
   namespace ppc {                                                                                 \
   template hipError_t launch_flatten<TYPE>(const FlatArgs&, int, hipStream_t);                   \
+  template hipError_t launch_mech_eval<TYPE>(int, const double*, const double*, int, double*,     \
+                                             hipStream_t);                                        \
   }
 
 #define PPC_INSTANTIATE_ADJOINT(TYPE, MODE)                                                           \

@@ -18,6 +18,7 @@ struct AdaptParamsDev {  // adaptive BDF tolerances
   int max_steps;
   int lu_batch, lu_wait;  // batched forms: wave-shared Jacobian/LU service (0: defaults 64, 32)
   int lane_occ;           // batched forms: kernel build held to 1 or 2 waves per SIMD (0: 2)
+  int group;              // batched forms: cells per wave of the group kernel (8, 4, 2, 1); 0: one cell per lane
 };
 
 struct ReactParamsDev {

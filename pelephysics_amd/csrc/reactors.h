@@ -421,6 +421,11 @@ PPC_HEAVY void change_D(double* __restrict__ D, const int order, const double fa
   }
 }
 
+// RHS evaluations a cell of the adaptive integrators may use: a bound on the
+// work of one cell (16 per allowed step) whatever the arithmetic does, so that
+// every per-cell loop ends.
+PPC_DI long long rhs_budget(int max_steps) { return 16LL * max_steps + 256; }
+
 // Adaptive BDF for one cell from t = 0 to t_end; y (n) in/out.  LU_T is the
 // precision of the Newton-matrix factorisation (double: plain FP64 form).
 // Returns the accepted step count (>= 0) or a negative status.
@@ -470,6 +475,7 @@ PPC_DI int adaptive_cell(const AdaptParamsDev& p, const int reactor_type, double
   bdf_jacobian<M>(reactor_type, y0, 1.0, J, J);
   njac++;
   int status = 0;
+  const long long rhs_cap = rhs_budget(p.max_steps);
   while (t < t_bound) {
     if (nsteps >= p.max_steps) {
       status = -2;
@@ -485,8 +491,12 @@ PPC_DI int adaptive_cell(const AdaptParamsDev& p, const int reactor_type, double
     int n_iter = 0;
     double t_new = t, error_norm = 0.0, safety = 0.9;
     while (!accepted) {
-      if (h_abs < min_step) {
+      if (!(h_abs >= min_step)) {  // also ends a cell whose step size is NaN
         status = -3;
+        break;
+      }
+      if (nrhs >= rhs_cap) {
+        status = -2;
         break;
       }
       t_new = t + h_abs;

@@ -66,7 +66,7 @@ class Config(C.Structure):
         ("max_steps", C.c_int), ("block_size", C.c_int), ("workspace_bytes", C.c_longlong),
         ("clean_init_massfrac", C.c_int), ("batched_waves_per_cu", C.c_int),
         ("batched_lu_batch", C.c_int), ("batched_lu_wait", C.c_int), ("batched_box_direct", C.c_int),
-        ("batched_waves_per_simd", C.c_int),
+        ("batched_waves_per_simd", C.c_int), ("batched_cells_per_wave", C.c_int),
     ]
 
 
@@ -100,6 +100,7 @@ _KEYS = {
     "clean_init_massfrac": "clean_init_massfrac", "batched_waves_per_cu": "batched_waves_per_cu",
     "batched_lu_batch": "batched_lu_batch", "batched_lu_wait": "batched_lu_wait",
     "batched_box_direct": "batched_box_direct", "batched_waves_per_simd": "batched_waves_per_simd",
+    "batched_cells_per_wave": "batched_cells_per_wave",
 }
 
 _lib = None
@@ -144,6 +145,9 @@ def library() -> C.CDLL:
                                         vp, vp, vp, C.c_double, vp]
         L.pp_chem_react_1d_chained.argtypes = [vp, vp, vp, vp, vp, C.POINTER(C.c_double),
                                                 C.POINTER(C.c_double), C.c_int, C.c_int, vp]
+        L.pp_chem_production_rate.argtypes = [vp, vp, vp, C.c_int, vp, vp]
+        L.pp_chem_ajacobian.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, vp]
+        L.pp_chem_progress_rate.argtypes = [vp, vp, vp, C.c_int, vp, vp]
         L.pp_chem_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.pp_chem_copy_cell_steps.argtypes = [vp, vp, C.c_int]
         L.pp_chem_last_error.argtypes = [vp]
@@ -158,6 +162,23 @@ def mechanisms() -> Sequence[str]:
     buf = C.create_string_buffer(n)
     L.pp_chem_list_mechanisms(buf, n)
     return buf.value.decode().split(",")
+
+
+def _device_f64(t, name: str, numel: Optional[int] = None):
+    """Raise ValueError unless t is a float64 tensor on a GPU (and, when
+    numel is given, holds exactly that many elements).  The library takes raw
+    device addresses, so a tensor of another dtype, a host tensor or a short
+    buffer would be read and written out of bounds by the kernels."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if t.dtype != torch.float64:
+        raise ValueError(f"{name}: expected float64, got {t.dtype}")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name}: expected a GPU tensor, got device {t.device}")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"{name}: expected {numel} elements, got {t.numel()}")
+    return t
 
 
 def _array4_from_tensor(t, ncomp_axis_first=True) -> Array4:
@@ -210,7 +231,7 @@ class ReactorBase:
             if entry:
                 mechanism = entry["kernels"]
                 for k in ("batched_waves_per_cu", "batched_lu_batch", "batched_lu_wait", "batched_box_direct",
-                          "batched_waves_per_simd"):
+                          "batched_waves_per_simd", "batched_cells_per_wave"):
                     if k in entry:
                         ode.setdefault(k, entry[k])
         cfg = Config()
@@ -273,6 +294,7 @@ class ReactorBase:
     def react(self, rY, rYsrc, T, rE, rEsrc, FC, dt_react: float, time: float = 0.0,
               mask=None, stream: Optional[int] = None):
         """Box form on device tensors shaped (ncomp, nz, ny, nx); returns (avg_steps, time)."""
+        self._box_tensors(rY=rY, rYsrc=rYsrc, T=T, rE=rE, rEsrc=rEsrc, FC=FC)
         nz, ny, nx = T.shape[-3:]
         lo = (C.c_int * 3)(0, 0, 0)
         hi = (C.c_int * 3)(nx - 1, ny - 1, nz - 1)
@@ -284,11 +306,29 @@ class ReactorBase:
                                          None, C.byref(dt), C.byref(tm), stream)
         return self._check(rc), tm.value
 
+    def _box_tensors(self, **views):
+        """Every box view is a float64 GPU tensor over the same (nz, ny, nx)
+        box as T, with the species count of components for rY / rYsrc."""
+        T = views["T"]
+        _device_f64(T, "T")
+        box = tuple(T.shape[-3:])
+        for name, t in views.items():
+            if t is None and name == "FC":
+                continue
+            _device_f64(t, name)
+            if t.dim() not in (3, 4) or tuple(t.shape[-3:]) != box:
+                raise ValueError(f"{name}: expected shape (ncomp, {box[0]}, {box[1]}, {box[2]}), got {tuple(t.shape)}")
+            ncomp = 1 if t.dim() == 3 else t.shape[0]
+            need = self.nspec if name in ("rY", "rYsrc") else 1
+            if ncomp < need:
+                raise ValueError(f"{name}: expected at least {need} components, got {ncomp}")
+
     def flatten(self, rY, rYsrc, T, rE, rEsrc, stream: Optional[int] = None):
         """Box -> flat (YCOrder) state of device tensors shaped (ncomp, nz, ny, nx)
         (ReactorBase::flatten); returns torch tensors (y_vect, src_vect,
         vect_energy, src_vect_energy) on the same device."""
         import torch
+        self._box_tensors(rY=rY, rYsrc=rYsrc, T=T, rE=rE, rEsrc=rEsrc)
         nz, ny, nx = T.shape[-3:]
         ncells, ns = nx * ny * nz, self.nspec
         lo = (C.c_int * 3)(0, 0, 0)
@@ -307,12 +347,18 @@ class ReactorBase:
         """Flat -> box state (ReactorBase::unflatten): rY, T, rE, FC updated in
         place; FCunt is an int64 device tensor of per-cell step counts."""
         import torch
+        self._box_tensors(rY=rY, T=T, rE=rE, rEsrc=rEsrc, FC=FC)
         nz, ny, nx = T.shape[-3:]
+        ncells = nx * ny * nz
+        _device_f64(y_vect, "y_vect", ncells * (self.nspec + 1))
+        _device_f64(vect_energy, "vect_energy", ncells)
         lo = (C.c_int * 3)(0, 0, 0)
         hi = (C.c_int * 3)(nx - 1, ny - 1, nz - 1)
         views = [_array4_from_tensor(t) for t in (rY, T, rE, rEsrc, FC)]
-        if FCunt.dtype != torch.int64:
-            raise ValueError("FCunt must be int64")
+        if not isinstance(FCunt, torch.Tensor) or FCunt.dtype != torch.int64 or FCunt.device.type != "cuda":
+            raise ValueError("FCunt must be an int64 GPU tensor")
+        if FCunt.numel() != ncells:
+            raise ValueError(f"FCunt: expected {ncells} elements, got {FCunt.numel()}")
         rc = library().pp_chem_unflatten(self._h, lo, hi, *[C.byref(v) for v in views], y_vect.data_ptr(),
                                          vect_energy.data_ptr(), FCunt.data_ptr(), C.c_double(dt_react),
                                          stream)
@@ -322,16 +368,17 @@ class ReactorBase:
                  stream: Optional[int] = None):
         """1-D form: torch device tensors (in place) or numpy host arrays (copied)."""
         ncells = int(rX.shape[0])
+        sizes = self._sizes_1d(ncells)
         if isinstance(y, np.ndarray):
-            for a in (y, ysrc, rX, rXsrc):
-                if a.dtype != np.float64 or not a.flags.c_contiguous:
-                    raise ValueError("host arrays must be contiguous float64")
+            for a, (name, size) in zip((y, ysrc, rX, rXsrc), sizes):
+                if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags.c_contiguous:
+                    raise ValueError(f"{name}: host arrays must be contiguous float64 numpy arrays")
+                if a.size != size:
+                    raise ValueError(f"{name}: expected {size} elements, got {a.size}")
             ptrs = [a.ctypes.data for a in (y, ysrc, rX, rXsrc)]
             on_dev = 0
         else:
-            for a in (y, ysrc, rX, rXsrc):
-                if not a.is_contiguous():
-                    raise ValueError("device tensors must be contiguous")
+            self._device_1d((y, ysrc, rX, rXsrc), sizes)
             ptrs = [a.data_ptr() for a in (y, ysrc, rX, rXsrc)]
             on_dev = 1
         dt = C.c_double(dt_react)
@@ -340,13 +387,26 @@ class ReactorBase:
                                         stream)
         return self._check(rc), tm.value
 
+    def _sizes_1d(self, ncells: int):
+        """(name, element count) of the YCOrder arrays y, ysrc, rX, rXsrc."""
+        ns = self.nspec
+        return (("y", ncells * (ns + 1)), ("ysrc", ncells * ns), ("rX", ncells), ("rXsrc", ncells))
+
+    def _device_1d(self, arrays, sizes):
+        for a, (name, size) in zip(arrays, sizes):
+            _device_f64(a, name, size)
+            if not a.is_contiguous():
+                raise ValueError(f"{name}: device tensors must be contiguous")
+
     def react_1d_chained(self, y, ysrc, rX, rXsrc, dt_react: float, ncalls: int, time: float = 0.0,
                          stream: Optional[int] = None):
         """ncalls successive 1-D calls on device tensors, replayed from a HIP
-        graph of one call; returns (avg steps of the last call, time)."""
+        graph of one call; returns (avg steps of the last call, time).  Raises
+        ReactorError (EINTEGRATE) when a cell failed in any of the calls."""
         for a in (y, ysrc, rX, rXsrc):
-            if isinstance(a, np.ndarray) or not a.is_contiguous():
+            if isinstance(a, np.ndarray):
                 raise ValueError("chained calls take contiguous device tensors")
+        self._device_1d((y, ysrc, rX, rXsrc), self._sizes_1d(int(rX.shape[0])))
         dt = C.c_double(dt_react)
         tm = C.c_double(time)
         rc = library().pp_chem_react_1d_chained(self._h, y.data_ptr(), ysrc.data_ptr(), rX.data_ptr(),
@@ -367,14 +427,53 @@ class ReactorBase:
         sa, sb = (names.index(k) for k in species)
         nr = library().pp_chem_num_reactions(self._h)
         ncells = int(rX.shape[0])
-        if not (y.is_contiguous() and rX.is_contiguous()):
-            raise ValueError("device tensors must be contiguous")
+        self._device_1d((y, rX), (("y", ncells * (self.nspec + 1)), ("rX", ncells)))
         out = torch.empty((ncells, 3), dtype=torch.float64, device=y.device)
         grad = torch.empty((ncells, nr, 3), dtype=torch.float64, device=y.device)
         rc = library().pp_chem_adjoint_1d(self._h, y.data_ptr(), rX.data_ptr(), ncells, nmacro, dt_react, sa, sb,
                                           out.data_ptr(), grad.data_ptr(), stream)
         self._check(rc)
         return out, grad
+
+    def production_rate(self, sc, T, stream: Optional[int] = None):
+        """productionRate (Mechanisms/<mech>/mechanism.H) of the handle's
+        mechanism on the GPU: sc (ncells, N) in mol/m^3 and T (ncells,) in K,
+        float64 device tensors; returns wdot (ncells, N) in mol/m^3/s."""
+        import torch
+        ncells = int(T.shape[0])
+        _device_f64(sc, "sc", ncells * self.nspec)
+        _device_f64(T, "T", ncells)
+        w = torch.empty((ncells, self.nspec), dtype=torch.float64, device=T.device)
+        self._check(library().pp_chem_production_rate(self._h, sc.data_ptr(), T.data_ptr(), ncells,
+                                                      w.data_ptr(), stream))
+        return w
+
+    def progress_rate(self, sc, T, stream: Optional[int] = None):
+        """Net rates of progress q_f - q_r (comp_qfqr of Mechanisms/<mech>/mechanism.H)
+        on the GPU; returns q (ncells, NR) in mol/m^3/s."""
+        import torch
+        ncells = int(T.shape[0])
+        _device_f64(sc, "sc", ncells * self.nspec)
+        _device_f64(T, "T", ncells)
+        nr = library().pp_chem_num_reactions(self._h)
+        q = torch.empty((ncells, nr), dtype=torch.float64, device=T.device)
+        self._check(library().pp_chem_progress_rate(self._h, sc.data_ptr(), T.data_ptr(), ncells, q.data_ptr(),
+                                                    stream))
+        return q
+
+    def ajacobian(self, sc, T, consP: int, stream: Optional[int] = None):
+        """aJacobian (Mechanisms/<mech>/mechanism.H) on the GPU; returns J
+        (ncells, N+1, N+1) with J[c, k, m] = d wdot_m / d c_k (column-major
+        per cell, as the reference's J[k*(N+1)+m])."""
+        import torch
+        ncells = int(T.shape[0])
+        n = self.nspec + 1
+        _device_f64(sc, "sc", ncells * self.nspec)
+        _device_f64(T, "T", ncells)
+        J = torch.empty((ncells, n, n), dtype=torch.float64, device=T.device)
+        self._check(library().pp_chem_ajacobian(self._h, sc.data_ptr(), T.data_ptr(), ncells, int(consP),
+                                                J.data_ptr(), stream))
+        return J
 
     def stats(self) -> dict:
         s = Stats()

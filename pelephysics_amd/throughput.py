@@ -57,6 +57,8 @@ def main(argv=None) -> dict:
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--lu-batch", type=int, default=0)
     ap.add_argument("--lu-wait", type=int, default=0)
+    ap.add_argument("--cells-per-wave", type=int, default=0,
+                    help="batched forms: 0 one cell per lane; 8, 4, 2, 1 a group of 64/k lanes per cell")
     a = ap.parse_args(argv)
 
     import torch
@@ -65,7 +67,8 @@ def main(argv=None) -> dict:
     dev = torch.device("cuda:0")
     R = ReactorBase.create(a.integrator)
     R.init(2, st.ncells, mechanism=a.mech, rtol=a.rtol, atol=a.atol, batched_waves_per_cu=a.waves_per_cu,
-           batched_lu_batch=a.lu_batch, batched_lu_wait=a.lu_wait, batched_waves_per_simd=a.waves_per_simd)
+           batched_lu_batch=a.lu_batch, batched_lu_wait=a.lu_wait, batched_waves_per_simd=a.waves_per_simd,
+           batched_cells_per_wave=a.cells_per_wave)
     y0 = torch.from_numpy(st.y_1d()).to(dev)
     x0 = torch.from_numpy(st.rE.copy()).to(dev)
     y, x = y0.clone(), x0.clone()

@@ -11,7 +11,13 @@ fixed temperature grid and writes, per mechanism, a JSON fixture with:
 * ``reactions``: for each ``productionRate`` block, the forward rate constant
   k_f(T), the low-pressure rate k_0(T) and the Troe F_cent(T) on the grid, the
   third-body enhancement (efficiency - 1 per species), reactant and net
-  stoichiometry, reversibility and the power of ``refC`` in the reverse rate.
+  stoichiometry, reversibility and the power of ``refC`` in the reverse rate;
+  for falloff blocks also ``falloff``: per grid temperature, pairs
+  (log10 Pr, falloff factor) where the factor is the block's own statement
+  chain from ``redP`` to the final ``Corr`` (F * F_troe for Troe blocks,
+  redP / (1 + redP) for Lindemann blocks) evaluated with redP = Pr, at
+  log10 Pr = -troe_c + {-2, -1, 0, 1, 2} (troe_c of the block at that T;
+  Lindemann blocks at log10 Pr = -2..2).
 
 Run from the repository root, where ``/root/reference`` exists:
 
@@ -111,6 +117,39 @@ def _production_blocks(text: str):
     return blocks
 
 
+def _falloff_chain(flat: str):
+    """(log10 Pr, factor) pairs on the grid from the block's statements after redP."""
+    seg = flat[flat.index("redP ="):]
+    stmts = re.findall(r"(?:const amrex::Real )?(\w+) =\s*(.*?);", seg)
+    assert stmts[0][0] == "redP"
+    chain = []
+    for name, expr in stmts[1:]:
+        chain.append((name, expr))
+        if name == "Corr":
+            break
+    else:
+        raise ValueError("falloff block without a final Corr statement")
+    troe_c = next((e for n, e in chain if n == "troe_c"), None)
+    logfc = next((e for n, e in chain if n == "logFcent"), None)
+    out = []
+    for T in TGRID:
+        env = {"T": T, "invT": 1.0 / T, "logT": math.log(T)}
+        if troe_c is not None:
+            env["logFcent"] = _eval(logfc, env)
+            mid = -_eval(troe_c, env)
+        else:
+            mid = 0.0
+        pts = []
+        for off in (-2.0, -1.0, 0.0, 1.0, 2.0):
+            e = dict(env)
+            e["redP"] = 10.0 ** (mid + off)
+            for name, expr in chain:
+                e[name] = _eval(expr, e)
+            pts.append([mid + off, e["Corr"]])
+        out.append(pts)
+    return out
+
+
 def extract(mech: str) -> dict:
     with open(os.path.join(REF, mech, "mechanism.H")) as f:
         text = f.read()
@@ -144,6 +183,10 @@ def extract(mech: str) -> dict:
             r["k0"] = [_eval(mr.group(1), {"T": T, "invT": 1 / T, "logT": math.log(T)}) for T in TGRID]
         else:
             r["k0"] = None
+        if mr:
+            r["falloff"] = _falloff_chain(flat)
+        else:
+            r["falloff"] = None
         mf = re.search(r"logFcent = log10\((.*?)\);", flat)
         if mf:
             r["Fcent"] = [_eval(mf.group(1), {"T": T, "invT": 1 / T, "logT": math.log(T)}) for T in TGRID]

@@ -11,8 +11,8 @@ Workloads: dodecane cells drawn from the uneven-stiffness state (70% cold
 fresh gas at 300-700 K, 25% near ignition at 900-1200 K and 1-40 atm, 5% burnt
 at 2300 K, shuffled); heptane cells near ignition (800-1300 K, phi 0.5-2,
 10-50 atm).  Criteria as in test_gpu_parity: |x - x_ref| <= 1e-6 |x_ref| +
-1e-10 on T and Y_k, widened by the oracle's rounding spread for the
-fixed-step BDF, exact per-cell step counts for the fixed-step and explicit
+1e-10 on T and Y_k, within the hull of the oracle's rounding ensemble for
+the fixed-step BDF, exact per-cell step counts for the fixed-step and explicit
 integrators.
 """
 import numpy as np
@@ -20,7 +20,7 @@ import pytest
 
 import oracle
 from pelephysics_amd import workloads as W
-from test_gpu_parity import _torch, _close, _compare_y, _run_1d, _oracle_1d, _spread_1d, _TY
+from test_gpu_parity import _torch, _close, _compare_y, _run_1d, _oracle_1d, _hull_1d, _TY
 
 pytestmark = pytest.mark.gpu
 
@@ -34,8 +34,9 @@ def _case(models, mech, ncells=None, seed=W.SEED):
     return m, st, np.zeros((st.ncells, m.n_species)), np.zeros(st.ncells)
 
 
-@pytest.mark.parametrize("mech", ["dodecane_lu", "heptane_lu_88sk"])
-def test_large_bdf_parity(models, oracles, mech):
+# banded cells measured on the CPU: dodecane 8 of 64, heptane 10 of 24 (near ignition)
+@pytest.mark.parametrize("mech,max_banded", [("dodecane_lu", 0.15), ("heptane_lu_88sk", 0.45)])
+def test_large_bdf_parity(models, oracles, mech, max_banded):
     """ReactorBDF defaults (BDF3 with TRPZ ramp, SGS-preconditioned GMRES with
     kspiters = N + 1) at dt = 1e-5 s."""
     torch = _torch()
@@ -43,7 +44,7 @@ def test_large_bdf_parity(models, oracles, mech):
     ret, y, rX, R = _run_1d(torch, mech, "ReactorBDF", st, ysrc, esrc, 1e-5)
     oret, oy, orX, fc = _oracle_1d(oracles[mech], "bdf", st, ysrc, esrc, 1e-5)
     assert ret == oret == 1
-    _compare_y(m.n_species + 1, y, oy, _spread_1d(models, mech, st, ysrc, esrc, 1e-5))
+    _compare_y(m.n_species + 1, y, oy, _hull_1d(models, mech, st, ysrc, esrc, 1e-5), max_banded)
     _close("rX", rX, orX)
     assert (R.cell_steps(st.ncells) == fc).all()
 

@@ -8,19 +8,21 @@ explicit integrators.
 The fixed-step ReactorBDF (one linearised Newton iteration per step, GMRES
 with an absolute tolerance) amplifies rounding on some cells: two valid
 roundings of the same algorithm (the oracle built with and without fused
-multiply-adds) differ there by more than the criterion (up to 10% of T on hot
-drm19 cells in one 1e-5 s TRPZ step).  For that integrator the bound is
-widened per cell by 4x the oracle's own spread: the largest deviation from
-the reference result among the oracle built the other way, both builds run
-on inputs perturbed within the algorithm's own resolution -- rhoY scaled by
-1 +- 2^-52 and T moved by +-1e-6 K, the stopping tolerance of the
-temperature inversion that every right-hand side performs
-(Mechanisms/drm19/mechanism.H:2789-2890) -- and the oracle with every rate
-constant scaled by a random factor exp(eps), |eps| ~ 8 ulp, the size of the
-differences between device and host exp/log.  On some cells a change that
-small flips a discrete decision of the linear solve (the GPU result then
-equals, to the last digit, one of the two outcomes the oracle produces).
-Cells whose spread is below the criterion are held to the criterion itself.
+multiply-adds) differ there by more than the criterion (up to a third of T on
+hot drm19 cells in one 1e-5 s TRPZ step).  For that integrator the oracle is
+run as an ensemble of rounding variants: the reference computation, the
+oracle built the other way, both builds on inputs perturbed within the
+algorithm's own resolution -- rhoY scaled by 1 +- 2^-52 and T moved by
++-1e-6 K, the stopping tolerance of the temperature inversion that every
+right-hand side performs (Mechanisms/drm19/mechanism.H:2789-2890) -- and the
+oracle with every rate constant scaled by a random factor exp(eps),
+|eps| ~ 8 ulp, the size of the differences between device and host exp/log.
+The GPU value must lie in the ensemble's hull widened by the criterion,
+    min(ensemble) - tol <= x_gpu <= max(ensemble) + tol,
+which on a cell whose ensemble agrees to within the criterion is the
+criterion itself.  A cell is "banded" when its hull is wider than the
+criterion on some component; each test prints how many cells are banded and
+fails when more are than the fraction measured for its workload.
 """
 import numpy as np
 import pytest
@@ -39,11 +41,15 @@ def _torch():
     return torch
 
 
-def _close(name, got, ref, rtol=RTOL, atol=ATOL, band=None):
+def _close(name, got, ref, rtol=RTOL, atol=ATOL, hull=None):
+    """|got - ref| <= rtol |ref| + atol, or with hull = (lo, hi) of an
+    ensemble of rounding variants: lo - tol <= got <= hi + tol."""
     tol = rtol * np.abs(ref) + atol
-    if band is not None:
-        tol = tol + 4.0 * band
-    err = np.abs(got - ref) - tol
+    if hull is None:
+        err = np.abs(got - ref) - tol
+    else:
+        lo, hi = hull
+        err = np.maximum(lo - tol - got, got - hi - tol)
     bad = np.argwhere(err > 0)
     assert bad.size == 0, (f"{name}: {bad.shape[0]} entries out of tolerance, worst "
                            f"got={got.flat[np.argmax(err)]!r} ref={ref.flat[np.argmax(err)]!r}")
@@ -54,13 +60,25 @@ def _TY(n, y):
     return y[:, -1], y[:, :-1] / y[:, :-1].sum(1, keepdims=True)
 
 
-def _compare_y(n, got, ref, band=None):
-    """band: (T, Y) rounding-spread band of the oracle around ref."""
+def _banded(ref, hull, rtol=RTOL, atol=ATOL):
+    """Entries whose ensemble hull is wider than the criterion."""
+    lo, hi = hull
+    return (hi - lo) > rtol * np.abs(ref) + atol
+
+
+def _compare_y(n, got, ref, hull=None, max_banded=0.0):
+    """hull: ((loT, hiT), (loY, hiY)) of the oracle's rounding ensemble around
+    ref; at most max_banded of the cells may have a hull wider than the
+    criterion."""
     gT, gY = _TY(n, got)
     rT, rY = _TY(n, ref)
-    bT, bY = band if band is not None else (None, None)
-    _close("T", gT, rT, band=bT)
-    _close("Y", gY, rY, band=bY)
+    hT, hY = hull if hull is not None else (None, None)
+    if hull is not None:
+        cells = _banded(rT, hT) | _banded(rY, hY).any(1)
+        print(f"banded cells: {int(cells.sum())} of {cells.size} (at most {max_banded:.0%} allowed)")
+        assert cells.sum() <= max_banded * cells.size
+    _close("T", gT, rT, hull=hT)
+    _close("Y", gY, rY, hull=hY)
 
 
 def _state(models, mech, ncells, seed=7):
@@ -130,29 +148,34 @@ def _rounding_variants(models, mech, st):
             [(_rate_jittered(models[mech], seed), st) for seed in range(12)])
 
 
-def _spread_1d(models, mech, st, ysrc, esrc, dt, reactor_type=2, params=None):
-    """Per-entry rounding spread band of the fixed-step BDF oracle around its result."""
+def _hull_1d(models, mech, st, ysrc, esrc, dt, reactor_type=2, params=None):
+    """Per-entry hull ((loT, hiT), (loY, hiY)) of the fixed-step BDF oracle's
+    rounding ensemble (the reference computation and its variants)."""
     ref = _oracle_1d(oracle.Mech(models[mech]), "bdf", st, ysrc, esrc, dt, reactor_type, params)[1]
     n = models[mech].n_species + 1
     rT, rY = _TY(n, ref)
-    bT, bY = np.zeros_like(rT), np.zeros_like(rY)
+    loT, hiT, loY, hiY = rT.copy(), rT.copy(), rY.copy(), rY.copy()
     for O, s2 in _rounding_variants(models, mech, st):
         vT, vY = _TY(n, _oracle_1d(O, "bdf", s2, ysrc, esrc, dt, reactor_type, params)[1])
-        bT = np.maximum(bT, np.abs(vT - rT))
-        bY = np.maximum(bY, np.abs(vY - rY))
-    return bT, bY
+        loT, hiT = np.minimum(loT, vT), np.maximum(hiT, vT)
+        loY, hiY = np.minimum(loY, vY), np.maximum(hiY, vY)
+    return (loT, hiT), (loY, hiY)
 
 
-@pytest.mark.parametrize("mech,ncells", [("LiDryer", 200), ("drm19", 130)])
-def test_bdf_parity_defaults_1d(models, oracles, mech, ncells):
+# banded fractions measured on the CPU: LiDryer at 1e-5 s is one TRPZ step
+# through ignition (112 of 200 cells banded, in the minor species); at 1e-6 s
+# and on the fresh drm19 cells no cell is banded
+@pytest.mark.parametrize("mech,ncells,dt,max_banded", [("LiDryer", 200, 1e-5, 0.6), ("LiDryer", 200, 1e-6, 0.0),
+                                                       ("drm19", 130, 1e-5, 0.0)])
+def test_bdf_parity_defaults_1d(models, oracles, mech, ncells, dt, max_banded):
     """ReactorBDF defaults (BDF3 with TRPZ ramp, 1 substep, SGS GMRES), 1-D, device pointers."""
     torch = _torch()
     m, st = _state(models, mech, ncells)
     ysrc, esrc = np.zeros((st.ncells, m.n_species)), np.zeros(st.ncells)
-    ret, y, rX, R = _run_1d(torch, mech, "ReactorBDF", st, ysrc, esrc, 1e-5)
-    oret, oy, orX, fc = _oracle_1d(oracles[mech], "bdf", st, ysrc, esrc, 1e-5)
+    ret, y, rX, R = _run_1d(torch, mech, "ReactorBDF", st, ysrc, esrc, dt)
+    oret, oy, orX, fc = _oracle_1d(oracles[mech], "bdf", st, ysrc, esrc, dt)
     assert ret == oret == 1
-    _compare_y(m.n_species + 1, y, oy, _spread_1d(models, mech, st, ysrc, esrc, 1e-5))
+    _compare_y(m.n_species + 1, y, oy, _hull_1d(models, mech, st, ysrc, esrc, dt), max_banded)
     _close("rX", rX, orX)
     assert (R.cell_steps(st.ncells) == fc).all()
 
@@ -168,7 +191,7 @@ def test_bdf_parity_substeps_sources(models, oracles, precond):
     p = oracle.bdf_params(m.n_species, nsubsteps=40, gmres_precond=precond)
     oret, oy, orX, _ = _oracle_1d(oracles["LiDryer"], "bdf", st, ysrc, esrc, 1e-6, params=p)
     assert ret == oret == 40
-    _compare_y(m.n_species + 1, y, oy, _spread_1d(models, "LiDryer", st, ysrc, esrc, 1e-6, params=p))
+    _compare_y(m.n_species + 1, y, oy, _hull_1d(models, "LiDryer", st, ysrc, esrc, 1e-6, params=p))
     _close("rX", rX, orX)
 
 
@@ -190,6 +213,34 @@ def test_bdf_parity_failure_behaviour(models, oracles):
     assert f"{int((fc < 0).sum())} cell(s) failed" in str(e.value)
 
 
+def test_bdf_gmres_stale_restarts(models, oracles):
+    """A GMRES tolerance no cycle reaches (bdf_gmres_tol = 1e-30): every one of
+    the restarts repeats the first cycle's correction, because the reference
+    forms the residual once before its restart loop (ReactorBDFsolver.H:419-499;
+    the oracle is pinned to the closed form y0 + restarts * Mat^-1 rhs on
+    linear systems).  One BDF1 step of 1e-7 s on 64 LiDryer cells: the GPU's
+    increment with 3 restarts is 3x its increment with 1 restart, and the
+    3-restart state meets the criterion against the oracle."""
+    torch = _torch()
+    m, st = _state(models, "LiDryer", 64, seed=5)
+    ysrc, esrc = np.zeros((st.ncells, m.n_species)), np.zeros(st.ncells)
+    dt, n = 1e-7, m.n_species + 1
+    inc = {}
+    for restarts in (1, 3):
+        ret, y, rX, R = _run_1d(torch, "LiDryer", "ReactorBDF", st, ysrc, esrc, dt, bdf_scheme=0,
+                                bdf_gmres_tol=1e-30, bdf_gmres_restarts=restarts)
+        assert ret == 1
+        inc[restarts] = (y - st.y_1d()).reshape(-1, n)
+    # (a restart from the true residual would give 1x: the ratio is pinned to 1e-5,
+    # the rounding of three Krylov cycles that stop at the 1e-30 tolerance)
+    err = np.abs(inc[3] - 3 * inc[1]).max(1) / np.abs(inc[1]).max(1)
+    assert err.max() < 1e-5, err.max()
+    p = oracle.bdf_params(m.n_species, scheme=0, gmres_tol=1e-30, gmres_restarts=3)
+    oret, oy, orX, fc = _oracle_1d(oracles["LiDryer"], "bdf", st, ysrc, esrc, dt, params=p)
+    assert oret == 1
+    _compare_y(n, y, oy, _hull_1d(models, "LiDryer", st, ysrc, esrc, dt, params=p))
+
+
 @pytest.mark.parametrize("scheme,iters", [(0, 1), (1, 3), (2, 2)])
 def test_bdf_parity_schemes_e_reactor(models, oracles, scheme, iters):
     """BDF1 / TRPZ / BDF2 with several Newton iterations, constant-volume (e) reactor."""
@@ -204,7 +255,7 @@ def test_bdf_parity_schemes_e_reactor(models, oracles, scheme, iters):
     oret, oy, orX, _ = _oracle_1d(oracles["LiDryer"], "bdf", st, ysrc, esrc, 2e-6, reactor_type=1, params=p)
     assert ret == oret
     _compare_y(m.n_species + 1, y, oy,
-               _spread_1d(models, "LiDryer", st, ysrc, esrc, 2e-6, reactor_type=1, params=p))
+               _hull_1d(models, "LiDryer", st, ysrc, esrc, 2e-6, reactor_type=1, params=p))
 
 
 @pytest.mark.parametrize("host", [False, True])
@@ -339,17 +390,25 @@ def test_box_form_ragged_halo(models, oracles, ident, mode):
     p = oracle.adapt_params(rtol=1e-10, atol=1e-16) if mode == "adaptive" else None
     oret, oFC = oracles["drm19"].react_box(mode, oY, np.zeros_like(oY), oT, orE, np.zeros(105), 1e-5,
                                            params=p)
-    bT = bY = None
+    gY = g_rY / g_rY.sum(1, keepdims=True)
+    rYm = oY.T / oY.T.sum(1, keepdims=True)
     if mode == "bdf":
-        bT, bY = np.zeros(105), np.zeros((105, m.n_species))
+        # hull of the rounding ensemble; 34 of the 105 cells are banded (measured on the CPU)
+        loT, hiT, loY, hiY = oT.copy(), oT.copy(), rYm.copy(), rYm.copy()
         for O2, s2 in _rounding_variants(models, "drm19", st):
             oY2, oT2, orE2 = np.ascontiguousarray(s2.rY.T).copy(), s2.T.copy(), s2.rE.copy()
             O2.react_box(mode, oY2, np.zeros_like(oY2), oT2, orE2, np.zeros(105), 1e-5)
-            bT = np.maximum(bT, np.abs(oT - oT2))
-            bY = np.maximum(bY, np.abs(oY.T / oY.T.sum(1, keepdims=True)
-                                       - oY2.T / oY2.T.sum(1, keepdims=True)))
-    _close("T", g_T, oT, band=bT)
-    _close("Y", g_rY / g_rY.sum(1, keepdims=True), oY.T / oY.T.sum(1, keepdims=True), band=bY)
+            vY = oY2.T / oY2.T.sum(1, keepdims=True)
+            loT, hiT = np.minimum(loT, oT2), np.maximum(hiT, oT2)
+            loY, hiY = np.minimum(loY, vY), np.maximum(hiY, vY)
+        cells = _banded(oT, (loT, hiT)) | _banded(rYm, (loY, hiY)).any(1)
+        print(f"banded cells: {int(cells.sum())} of 105")
+        assert cells.sum() <= 0.35 * 105
+        _close("T", g_T, oT, hull=(loT, hiT))
+        _close("Y", gY, rYm, hull=(loY, hiY))
+    else:
+        _close("T", g_T, oT)
+        _close("Y", gY, rYm)
     if mode == "bdf":
         assert rc == oret == 1
         assert (FC[inner].cpu().numpy().reshape(-1) == oFC).all()

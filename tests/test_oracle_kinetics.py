@@ -3,7 +3,12 @@
 * per reaction, at the fixture temperatures: forward rate = k_f * prod c^nu'
   (elementary), k_f * alpha * prod c^nu' (three-body, alpha from the literal
   enhancement), and at the Troe midpoint log Pr = -c the factor F = F_cent
-  (falloff); detailed balance q = 0 when prod c''^nu'' / prod c'^nu' equals
+  (falloff);
+* every falloff reaction at five reduced pressures around the Troe midpoint
+  (log Pr = -c - 2 .. -c + 2), where the troe_n and 0.14 terms of the
+  broadening factor matter, against the factor the reference's own statement
+  chain gives (the ``falloff`` entries of the fixture);
+* detailed balance q = 0 when prod c''^nu'' / prod c'^nu' equals
   K_c formed from the literal g/RT and the reference's reference
   concentration 101325 / 8.31446 / T;
 * element and mass conservation of the production rates;
@@ -77,6 +82,63 @@ def test_net_rates_against_literals(models, oracles, mech):
             q = O.qdot(c, T)[j]
             # the literal g/RT carry ~1e-9 relative rounding: K_c to ~1e-7
             assert q == pytest.approx(expect, rel=1e-6, abs=1e-10 * corr * kf * phi_f), (R.equation, T)
+
+
+def falloff_cases(m, d, rng, tis=range(8)):
+    """(j, T, c, expected q) at the literal falloff points of every falloff
+    reaction: c scaled so that the reduced pressure is the fixture's Pr."""
+    out = []
+    for r in d["reactions"]:
+        if r["falloff"] is None:
+            continue
+        j = r["index"]
+        R = m.reactions[j]
+        for ti in tis:
+            T = d["tgrid"][ti]
+            kf, k0 = r["kf"][ti], r["k0"][ti]
+            for logpr, factor in r["falloff"][ti]:
+                c = _conc(rng, m.n_species)
+                alpha = r["mixture_coef"] * c.sum() + sum(v * c[int(k)] for k, v in r["enhancement"].items())
+                c *= (10 ** logpr * kf / k0) / alpha
+                expect = factor * kf * np.prod([c[int(k)] ** v for k, v in r["reactants"].items()])
+                if R.reversible:
+                    expect -= factor * kf / _kc(R, d, ti) * np.prod([c[k] ** v for k, v in R.products.items()])
+                out.append((j, T, c, expect, factor * kf))
+    return out
+
+
+@pytest.mark.parametrize("mech", MECHS)
+def test_falloff_off_midpoint_against_literals(models, oracles, mech):
+    """Troe and Lindemann factors away from the Troe midpoint, from the
+    reference's literal statement chain (mechanism.H, e.g. drm19 :4446-4461)."""
+    m, O, d = models[mech], oracles[mech], _fix(mech)
+    cases = falloff_cases(m, d, np.random.default_rng(6))
+    assert len(cases) == 40 * sum(r["falloff"] is not None for r in d["reactions"])
+    for j, T, c, expect, scale in cases:
+        phi = np.prod([c[k] ** v for k, v in m.reactions[j].reactants.items()])
+        q = O.qdot(c, T)[j]
+        assert q == pytest.approx(expect, rel=1e-6, abs=1e-9 * scale * phi), (m.reactions[j].equation, T)
+
+
+def test_falloff_fixture_moves_off_midpoint():
+    """The off-midpoint points change the Troe factor by more than the test
+    tolerance when troe_n or the 0.14 coefficient changes (what a dropped or
+    wrong term would do): recompute one drm19 block's chain with n + 0.1."""
+    d = _fix("drm19")
+    r = next(x for x in d["reactions"] if x["Fcent"] is not None)
+    ti = 4
+    logfc = math.log10(r["Fcent"][ti])
+    c = -0.4 - 0.67 * logfc
+    for (logpr, factor), off in zip(r["falloff"][ti], (-2, -1, 0, 1, 2)):
+        Pr = 10 ** logpr
+        for dn, d14 in ((0.0, 0.0), (0.1, 0.0), (0.0, 0.02)):
+            n = 0.75 - 1.27 * logfc + dn
+            troe = (c + logpr) / (n - (0.14 + d14) * (c + logpr))
+            f = Pr / (1 + Pr) * 10 ** (logfc / (1 + troe * troe))
+            if dn == 0.0 and d14 == 0.0:
+                assert f == pytest.approx(factor, rel=1e-12)
+            elif off != 0:
+                assert abs(f / factor - 1) > 1e-5
 
 
 @pytest.mark.parametrize("mech", MECHS)

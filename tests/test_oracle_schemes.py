@@ -68,6 +68,45 @@ def test_bdf_parity_linear_closed_form(scheme, precond):
         np.testing.assert_allclose(y, ref, rtol=1e-8, atol=1e-10)
 
 
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_gmres_restart_from_stale_residual(seed):
+    """Restarted GMRES as the reference runs it (ReactorBDFsolver.H:419-499):
+    the preconditioned residual r is formed once, before the restart loop, and
+    never updated, so every restart cycle repeats the first cycle's
+    correction.  With the full Krylov dimension (kspiters = KSPSIZE = n) one
+    cycle solves the system to rounding, but on a system scaled so that the
+    rounding residual stays above the absolute 1e-10 tolerance, every one of
+    the `restarts` cycles runs: one BDF1 step (x0 = 0, Mat = I/dt - A,
+    rhs = A y0 + b) gives y0 + restarts * Mat^-1 rhs.  A restart from the
+    true residual would give y0 + Mat^-1 rhs whatever the restart count."""
+    rng = np.random.default_rng(seed)
+    n, dt = 8, 1e-6
+    A = -np.diag(rng.uniform(1e4, 1e7, n)) + rng.uniform(-3e5, 3e5, (n, n))
+    b = rng.uniform(-1, 1, n) * 1e8
+    y0 = rng.uniform(0.5, 1.5, n)
+    exact = np.linalg.solve(np.eye(n) / dt - A, A @ y0 + b)
+    for restarts in (1, 3, 10):
+        r, y = oracle.bdf_linear(A, y0, dt, b=b, nsubsteps=1, scheme=0, gmres_precond=0, gmres_kspiters=n,
+                                 gmres_restarts=restarts)
+        assert r == 1
+        np.testing.assert_allclose(y - y0, restarts * exact, rtol=1e-10)
+
+
+def test_gmres_short_krylov_without_convergence_fails():
+    """kspiters < KSPSIZE without convergence: the reference back-substitutes
+    over all KSPSIZE rows of the Hessenberg matrix (optkspsize keeps its
+    initial KSPSIZE, ReactorBDFsolver.H:447,478), dividing by the zero
+    diagonal of the rows never filled; the solution is NaN and the step
+    fails.  The same system converges with kspiters = n (previous test)."""
+    rng = np.random.default_rng(0)
+    n, dt = 8, 1e-6
+    A = -np.diag(rng.uniform(1e4, 1e7, n)) + rng.uniform(-3e5, 3e5, (n, n))
+    b = rng.uniform(-1, 1, n) * 1e8
+    y0 = rng.uniform(0.5, 1.5, n)
+    r, y = oracle.bdf_linear(A, y0, dt, b=b, nsubsteps=1, scheme=0, gmres_precond=0, gmres_kspiters=3)
+    assert r < 0 or not np.isfinite(y).all()
+
+
 def test_bdf_truncated_coefficients_matter():
     """The recursion with exact 4/3, 1/3, 2/3 differs: the pin sees the truncation."""
     A, b, y0 = _linear_system()
