@@ -20,7 +20,12 @@ keeps the fastest:
   matrices together (16 lanes or 4 trips; 64 lanes or 8 trips; 64 lanes or
   32 trips, i.e. nearly always the whole wave at once);
 * batched_box_direct (batched forms, box calls): the box kernel itself (1) or
-  box -> flat state, the 1-D kernel, flat -> box (0).
+  box -> flat state, the 1-D kernel, flat -> box (0);
+* batched_cells_per_wave (batched forms, table-driven kernels): one cell per
+  lane (0) or a group of 8, 16, 32 or 64 lanes per cell (8, 4, 2, 1 cells per
+  wave; the sizes a mechanism's kernels are built for), with 8 or 4 waves
+  per CU and the LU service serving each group at once (1 cell, 1 trip) or
+  all of a wave's waiting groups together (64, 8 or 32 trips).
 
 Each candidate runs `calls` calls on the same cells and arrays; its time is
 the kernel time of the last one (the queue in the cost order of the call
@@ -43,9 +48,14 @@ import sys
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+# mechanisms compiled from their tables only, under their plain names
+TABLE_ONLY = ("dodecane_lu", "heptane_lu_88sk")
 BATCHED = ("ReactorBDFAdaptiveMixed", "ReactorBDFBatchedFP64")
 OCCUPANCY = ((2, 8), (2, 6), (2, 4), (1, 4), (1, 3))  # (waves per SIMD, waves per CU)
 LU_SERVICE = ((16, 4), (64, 8), (64, 32))
+GROUPS = (8, 4, 2, 1)  # cells per wave of the group kernel
+GROUP_WAVES = (0, 4)  # persistent waves per CU of the group kernel (0: every resident wave)
+GROUP_LU = ((1, 1), (64, 8), (64, 32))
 
 
 def arch() -> str:
@@ -78,10 +88,17 @@ def candidates(mech: str, integrator: str, form: str = "1d") -> list:
         return [{"kernels": k, "batched_waves_per_simd": 0, "batched_waves_per_cu": 0, "batched_lu_batch": 0,
                  "batched_lu_wait": 0, "batched_box_direct": 0} for k in impls]
     routes = (0, 1) if form == "box" else (0,)
-    return [{"kernels": k, "batched_waves_per_simd": o, "batched_waves_per_cu": w, "batched_lu_batch": b,
-             "batched_lu_wait": t, "batched_box_direct": r}
+    lane = [{"kernels": k, "batched_waves_per_simd": o, "batched_waves_per_cu": w, "batched_lu_batch": b,
+             "batched_lu_wait": t, "batched_box_direct": r, "batched_cells_per_wave": 0}
             for k in impls for o, w in OCCUPANCY for b, t in LU_SERVICE for r in routes
             if not (o == 1 and r == 1)]  # the box kernel is built at 2 waves per SIMD only
+    # group kernels: table-driven implementations only (a size a mechanism is
+    # not built for is rejected at init and skipped)
+    tables = [k for k in impls if k.endswith("_table") or k in TABLE_ONLY]
+    group = [{"kernels": k, "batched_waves_per_simd": 0, "batched_waves_per_cu": w, "batched_lu_batch": b,
+              "batched_lu_wait": t, "batched_box_direct": 0, "batched_cells_per_wave": g}
+             for k in tables for g in GROUPS for w in GROUP_WAVES for b, t in GROUP_LU]
+    return lane + group
 
 
 def _time(cand: dict, integrator: str, form: str, st, m, dt: float, rtol: float, atol: float,

@@ -423,11 +423,11 @@ class ReactorBase:
         grad (ncells, NR, 3) = d out / d ln A_j, j in mechanism order."""
         import torch
         from .mechgen.__main__ import load
+        ncells = int(rX.shape[0])
+        self._device_1d((y, rX), (("y", ncells * (self.nspec + 1)), ("rX", ncells)))
         names = [sp.name for sp in load(self.mechanism.replace("_table", "")).species]
         sa, sb = (names.index(k) for k in species)
         nr = library().pp_chem_num_reactions(self._h)
-        ncells = int(rX.shape[0])
-        self._device_1d((y, rX), (("y", ncells * (self.nspec + 1)), ("rX", ncells)))
         out = torch.empty((ncells, 3), dtype=torch.float64, device=y.device)
         grad = torch.empty((ncells, nr, 3), dtype=torch.float64, device=y.device)
         rc = library().pp_chem_adjoint_1d(self._h, y.data_ptr(), rX.data_ptr(), ncells, nmacro, dt_react, sa, sb,

@@ -221,3 +221,42 @@ def premixed(m: Mechanism, ncells: int, seed: int = SEED) -> CellState:
         p = rng.uniform(1.0, 40.0, ncells) * PATM
         return _finish(m, m.name, _fuel_air_Y(m, phi), T, p)
     return heptane_ignition(m, ncells, seed)
+
+
+def ignition_delay_state(m: Mechanism, T0: float = 600.0, p: float = 5066250.0,
+                         equiv_ratio: float = 7.928571428571429, ncells: int = 1) -> CellState:
+    """The homogeneous-reactor start of the reference's ignition-delay test
+    (Testing/Exec/IgnitionDelay/GPU_misc.H:59-65 with inputs.0d_firstpass):
+    X_O2 = 1 / (phi / 18.5 + 1 + 0.79 / 0.21), X_N2 = 0.79 X_O2 / 0.21, the
+    fuel the rest; T0 = 600 K, p = 5 atm, h-form."""
+    fuel, _ = FUELS[m.name]
+    xo2 = 1.0 / (equiv_ratio / 18.5 + 1.0 + 0.79 / 0.21)
+    xn2 = 0.79 * xo2 / 0.21
+    Y = _mass_from_moles(m, {"O2": xo2 * np.ones(ncells), "N2": xn2 * np.ones(ncells),
+                             fuel: (1.0 - xo2 - xn2) * np.ones(ncells)}, ncells)
+    return _finish(m, m.name, Y, np.full(ncells, T0), np.full(ncells, p))
+
+
+def ignition_delay(advance, T0: float, dt: float = 2.0, ndt: int = 2000):
+    """The ignition delay as the reference's test measures it
+    (Testing/Exec/IgnitionDelay/computeIgnitionDelay.py, exec_ignDelay.sh):
+    calls of dt / ndt from the start until T exceeds T0 + 400 K, the delay
+    being (dt / ndt) x that call count; first with dt = 2 s, ndt = 2000, then
+    again with dt = 1.1 x the first estimate and ndt = 4000.  `advance(start,
+    dt_call, ncalls)` integrates ncalls calls of dt_call (from the initial
+    state when start is true, else from where the previous calls ended) and
+    returns the temperature after each.  Returns (delay, uncertainty dt / ndt
+    of the refined pass)."""
+    def first_crossing(dtc, n):
+        done, chunk = 0, 50
+        while done < n:
+            k = min(chunk, n - done)
+            Ts = advance(done == 0, dtc, k)
+            hit = np.argwhere(np.asarray(Ts) > T0 + 400.0)
+            if hit.size:
+                return done + int(hit[0, 0]) + 1
+            done += k
+        raise RuntimeError("no ignition within the calls")
+    est = dt / ndt * first_crossing(dt / ndt, ndt)
+    dt2, ndt2 = est * 1.1, 2 * ndt
+    return dt2 / ndt2 * first_crossing(dt2 / ndt2, ndt2), dt2 / ndt2

@@ -1,7 +1,8 @@
 """The batched adaptive BDF with a group of lanes per cell
 (batched_cells_per_wave = 8, 4, 2, 1: 8, 16, 32 or 64 lanes per cell;
 group_bdf.h) against the tight FP64 oracle, at the benchmark tolerances
-(GPU rtol 1e-10 / atol 1e-16, oracle rtol 1e-13 / atol 1e-20):
+(GPU rtol 1e-10 / atol 1e-16, LiDryer 1e-12 / 1e-18; oracle rtol 1e-13 /
+atol 1e-20):
 
 * every cell meets |x - x_ref| <= 1e-6 |x_ref| + 1e-10 on T and Y_k, with
   the FP32 and the FP64 Newton-matrix LU, on every mechanism and group size
@@ -38,10 +39,17 @@ def _cells(models, mech):
     return m, W.heptane_ignition(m, 24)
 
 
+# LiDryer's 1000-1400 K cells ignite within the call and amplify the local
+# errors: at rtol 1e-10 every form, the plain FP64 one included, misses the
+# criterion by the same 18x, so LiDryer runs at rtol 1e-12 / atol 1e-18
+TOLS = {"LiDryer": (1e-12, 1e-18)}
+
+
 def _run(torch, kernels, ident, st, m, calls=2, **ode):
     from pelephysics_amd.reactor import ReactorBase
     R = ReactorBase.create(ident)
-    R.init(2, st.ncells, mechanism=kernels, rtol=1e-10, atol=1e-16, **ode)
+    rtol, atol = TOLS.get(kernels.replace("_table", ""), (1e-10, 1e-16))
+    R.init(2, st.ncells, mechanism=kernels, rtol=rtol, atol=atol, **ode)
     dev = torch.device("cuda:0")
     ty = torch.from_numpy(st.y_1d()).to(dev)
     ts = torch.zeros(st.ncells * m.n_species, dtype=torch.float64, device=dev)
