@@ -195,6 +195,21 @@ int pp_chem_ajacobian(pp_chem* h, const double* sc, const double* T, int ncells,
                       void* stream);
 int pp_chem_progress_rate(pp_chem* h, const double* sc, const double* T, int ncells, double* q, void* stream);
 
+/* pp_chem_react_1d on device arrays without any device allocation: no
+ * counterpart in the reference.  For callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same integration and results as pp_chem_react_1d with on_device = 1
+ * (the queue visits the cells in index order; the per-cell step counts are
+ * reduced, the other per-cell counters are not kept).  Its queue counter,
+ * step counts and integrator workspace live in one file-scope device array of
+ * 64 MiB per GPU, held until the process ends: at most 16,777,088 cells per
+ * call (larger ncells returns PP_CHEM_EINVAL); a mechanism / integrator whose
+ * workspace needs more than the array holds runs fewer resident blocks.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (other streams, other handles, other host threads) race. */
+int pp_chem_react_1d_noalloc(pp_chem* h, double* y, double* ysrc, double* rX, double* rXsrc, double* dt_react,
+                             double* time, int ncells, void* stream);
+
 int pp_chem_get_stats(const pp_chem* h, pp_chem_stats* out);
 int pp_chem_num_reactions(const pp_chem* h);
 

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -190,6 +191,9 @@ struct pp_chem {
   int max_blocks = 0;
   // failed cells over every call of a chain (pp_chem_react_1d_chained)
   int* d_fail = nullptr;
+  // this device's address of the file-scope scratch array of
+  // pp_chem_react_1d_noalloc (null when it could not be resolved)
+  unsigned char* scratch = nullptr;
 };
 
 namespace {
@@ -234,16 +238,24 @@ ppc::CellStats counters(pp_chem* h) {
   return st;
 }
 
-// Reduces the counters of the last launch and returns int(average steps) or an error.
+// Reduces the counters st of the last launch (into the 9-word buffer red) and
+// returns int(average steps) or an error.
+int finish_with(pp_chem* h, int ncells, hipStream_t s, float kernel_ms, const ppc::CellStats& st,
+                unsigned long long* red);
+
 int finish(pp_chem* h, int ncells, hipStream_t s, float kernel_ms) {
-  ppc::CellStats st = counters(h);
-  hipLaunchKernelGGL(init_stats, dim3(1), dim3(64), 0, s, h->d_red);
+  return finish_with(h, ncells, s, kernel_ms, counters(h), h->d_red);
+}
+
+int finish_with(pp_chem* h, int ncells, hipStream_t s, float kernel_ms, const ppc::CellStats& st,
+                unsigned long long* red) {
+  hipLaunchKernelGGL(init_stats, dim3(1), dim3(64), 0, s, red);
   const int blocks = ncells > 0 ? (ncells + 255) / 256 < 1024 ? (ncells + 255) / 256 : 1024 : 1;
   hipLaunchKernelGGL(reduce_stats, dim3(blocks), dim3(256), 0, s, st.nsteps, st.nrhs, st.njac, st.nlu,
-                     st.nnewton, st.nreject, ncells, h->d_red);
+                     st.nnewton, st.nreject, ncells, red);
   HIPCHK(h, hipGetLastError());
   unsigned long long r[9];
-  HIPCHK(h, hipMemcpyAsync(r, h->d_red, sizeof(r), hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipMemcpyAsync(r, red, sizeof(r), hipMemcpyDeviceToHost, s));
   HIPCHK(h, hipStreamSynchronize(s));
   pp_chem_stats& S = h->stats;
   S.ncells = ncells;
@@ -265,6 +277,33 @@ int finish(pp_chem* h, int ncells, hipStream_t s, float kernel_ms) {
 }
 
 bool is_batched(int mode) { return mode == ppc::MODE_BATCHED_MIXED || mode == ppc::MODE_BATCHED_FP64; }
+
+// ---- device scratch of pp_chem_react_1d_noalloc: one array per GPU, held
+// until the process ends.  Layout: [0, 256) queue counter, [256, 512) the
+// statistics reduction, then the per-cell step counts, then the integrator
+// workspace.
+constexpr size_t kScratchBytes = 64ull << 20;
+constexpr size_t kScratchCounts = 512;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Resolves this device's address of `scratch` once per device (the caller
+// has made h->device current); host code uses that address, never the
+// array's name.
+hipError_t scratch_address(int device, unsigned char** out) {
+  static std::mutex m;
+  static unsigned char* addr[64] = {};
+  std::lock_guard<std::mutex> lk(m);
+  *out = nullptr;
+  if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+  if (!addr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = static_cast<unsigned char*>(p);
+  }
+  *out = addr[device];
+  return hipSuccess;
+}
 
 // Per-lane workspace of mechanisms too large for private memory: one row of
 // doubles per lane and matrix element, for at most the resident waves per CU
@@ -513,6 +552,9 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
     pp_chem_destroy(h);
     return PP_CHEM_EHIP;
   }
+  // the device scratch of pp_chem_react_1d_noalloc: a failure here only
+  // disables that entry point (it reports the error)
+  if (scratch_address(h->device, &h->scratch) != hipSuccess) h->scratch = nullptr;
   *out = h;
   return PP_CHEM_OK;
 }
@@ -693,6 +735,66 @@ int pp_chem_react_1d(pp_chem* h, double* y, double* ysrc, double* rX, double* rX
   float ms = 0.f;
   HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
   const int r = finish(h, ncells, s, ms);
+  if (h->cfg.mod_reactor) *time += *dt_react;
+  return r;
+}
+
+int pp_chem_react_1d_noalloc(pp_chem* h, double* y, double* ysrc, double* rX, double* rXsrc, double* dt_react,
+                             double* time, int ncells, void* stream) {
+  if (!h) return PP_CHEM_EINVAL;
+  if (!dt_react || !time || ncells < 0) return fail(h, PP_CHEM_EINVAL, "bad argument");
+  if (ncells > 0 && (!y || !ysrc || !rX || !rXsrc)) return fail(h, PP_CHEM_EINVAL, "null array");
+  if (!h->scratch) return fail(h, PP_CHEM_EHIP, "device scratch of pp_chem_react_1d_noalloc unavailable");
+  const size_t counts = ((size_t)ncells * sizeof(int) + 255) & ~(size_t)255;
+  if (kScratchCounts + counts > kScratchBytes)
+    return fail(h, PP_CHEM_EINVAL,
+                "ncells = " + std::to_string(ncells) + " exceeds the device scratch of pp_chem_react_1d_noalloc (" +
+                    std::to_string((kScratchBytes - kScratchCounts) / sizeof(int)) + " cells at most)");
+  int* counter = reinterpret_cast<int*>(h->scratch);
+  unsigned long long* red = reinterpret_cast<unsigned long long*>(h->scratch + 256);
+  int* nsteps = reinterpret_cast<int*>(h->scratch + kScratchCounts);
+  unsigned char* ws = h->scratch + kScratchCounts + counts;
+  const size_t ws_bytes = kScratchBytes - kScratchCounts - counts;
+  // the integrator workspace in what is left: fewer resident blocks when
+  // it holds fewer than the device can run
+  ppc::ReactParamsDev P = h->P;
+  P.ws = nullptr;
+  P.ws_lanes = 0;
+  const bool group = is_batched(h->cfg.mode) && h->cfg.batched_cells_per_wave != 0;
+  const long long rows = group ? h->L->group_ws_rows(h->cfg.batched_cells_per_wave) : h->L->ws_rows(h->cfg.mode);
+  if (rows > 0 && ncells > 0) {
+    int ncu = 0;
+    HIPCHK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+    const long long per_cu =
+        (is_batched(h->cfg.mode) && (group || h->cfg.batched_waves_per_simd != 1)) ? 8 : 4;
+    const long long lanes_per_cell = group ? 64 / h->cfg.batched_cells_per_wave : 1;
+    long long blocks = std::min<long long>(((long long)ncells * lanes_per_cell + 63) / 64, per_cu * std::max(ncu, 1));
+    if (h->max_blocks > 0) blocks = std::min<long long>(blocks, h->max_blocks);
+    blocks = std::min<long long>(blocks, (long long)(ws_bytes / (64 * rows * sizeof(double))));
+    if (blocks < 1) return fail(h, PP_CHEM_EINVAL, "the device scratch holds no 64-lane block of this workspace");
+    P.ws = reinterpret_cast<double*>(ws);
+    P.ws_lanes = blocks * 64;
+  }
+  ppc::CellStats st{};
+  st.nsteps = nsteps;  // the other per-cell counters are not kept
+  hipStream_t s = (hipStream_t)stream;
+  HIPCHK(h, hipEventRecord(h->ev0, s));
+  if (ncells > 0) {
+    hipError_t e;
+    if (is_batched(h->cfg.mode)) {
+      hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, counter, 1);
+      const ppc::QueueArgs q{nullptr, counter, ncells, h->max_blocks};  // cells in index order
+      e = h->L->batched_1d(h->cfg.mode, P, y, ysrc, rX, rXsrc, *dt_react, q, st, s);
+    } else {
+      e = h->L->react_1d(h->cfg.mode, P, y, ysrc, rX, rXsrc, *dt_react, ncells, st, h->cfg.block_size, s);
+    }
+    if (e != hipSuccess) return hip_fail(h, e, "react_1d_noalloc launch");
+  }
+  HIPCHK(h, hipEventRecord(h->ev1, s));
+  HIPCHK(h, hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  const int r = finish_with(h, ncells, s, ms, st, red);
   if (h->cfg.mod_reactor) *time += *dt_react;
   return r;
 }

@@ -145,6 +145,8 @@ def library() -> C.CDLL:
                                         vp, vp, vp, C.c_double, vp]
         L.pp_chem_react_1d_chained.argtypes = [vp, vp, vp, vp, vp, C.POINTER(C.c_double),
                                                 C.POINTER(C.c_double), C.c_int, C.c_int, vp]
+        L.pp_chem_react_1d_noalloc.argtypes = [vp, vp, vp, vp, vp, C.POINTER(C.c_double),
+                                                C.POINTER(C.c_double), C.c_int, vp]
         L.pp_chem_production_rate.argtypes = [vp, vp, vp, C.c_int, vp, vp]
         L.pp_chem_ajacobian.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, vp]
         L.pp_chem_progress_rate.argtypes = [vp, vp, vp, C.c_int, vp, vp]
@@ -385,6 +387,23 @@ class ReactorBase:
         tm = C.c_double(time)
         rc = library().pp_chem_react_1d(self._h, *ptrs, C.byref(dt), C.byref(tm), ncells, on_dev,
                                         stream)
+        return self._check(rc), tm.value
+
+    def react_1d_noalloc(self, y, ysrc, rX, rXsrc, dt_react: float, time: float = 0.0,
+                         stream: Optional[int] = None):
+        """The 1-D form on device tensors through pp_chem_react_1d_noalloc: no
+        device allocation (a per-GPU static scratch array holds the queue,
+        step counts and workspace; concurrent calls on one GPU race).
+        Returns (avg steps, time)."""
+        for a in (y, ysrc, rX, rXsrc):
+            if isinstance(a, np.ndarray):
+                raise ValueError("react_1d_noalloc takes contiguous device tensors")
+        ncells = int(rX.shape[0])
+        self._device_1d((y, ysrc, rX, rXsrc), self._sizes_1d(ncells))
+        dt = C.c_double(dt_react)
+        tm = C.c_double(time)
+        rc = library().pp_chem_react_1d_noalloc(self._h, y.data_ptr(), ysrc.data_ptr(), rX.data_ptr(),
+                                                rXsrc.data_ptr(), C.byref(dt), C.byref(tm), ncells, stream)
         return self._check(rc), tm.value
 
     def _sizes_1d(self, ncells: int):
