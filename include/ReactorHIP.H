@@ -14,6 +14,9 @@
 // with the same names; ode.rtol / ode.atol / ode.max_steps for the adaptive
 // forms.  The mechanism name comes from ode.mechanism (default: the string of
 // the compiled PelePhysics mechanism passed as PP_CHEM_MECHANISM).
+// With PP_CHEM_REFERENCE_NAMES defined (a build that leaves the reference's
+// ReactorBDF.cpp / ReactorRK64.cpp out of PelePhysics), "ReactorBDF" and
+// "ReactorRK64" are registered too, so an unchanged inputs file selects them.
 #pragma once
 
 #include <string>
@@ -42,7 +45,7 @@ public:
     if (pp_chem_config_init(&m_cfg, mech.c_str(), MODE, reactor_type) != 0) {
       amrex::Abort("ReactorHIP: mechanism not compiled into libpele_chem_hip");
     }
-    pp.query("verbose", verbose);
+    pp.query("verbose", this->verbose);
     pp.query("bdf_nonlinear_iters", m_cfg.bdf_nonlinear_iters);
     pp.query("bdf_nonlinear_tol", m_cfg.bdf_nonlinear_tol);
     pp.query("bdf_nsubsteps", m_cfg.bdf_nsubsteps);
@@ -79,7 +82,11 @@ public:
     }
     const int rc = pp_chem_create(&m_cfg, &m_h);
     if (rc != 0) {
-      amrex::Abort("Wrong reactor type. Choose between 1 (e) or 2 (h).");
+      if (reactor_type != 1 && reactor_type != 2) {
+        amrex::Abort("Wrong reactor type. Choose between 1 (e) or 2 (h).");  // ReactorTypes.H:12-19
+      }
+      amrex::Abort("ReactorHIP: pp_chem_create failed with code " + std::to_string(rc) +
+                   (rc == PP_CHEM_EHIP ? " (no usable GPU)" : " (invalid ode.* configuration)"));
     }
     return 0;
   }
@@ -253,5 +260,24 @@ template class ReactorHIP<PP_CHEM_BDF, kReactorBDF_HIP>;
 template class ReactorHIP<PP_CHEM_RK64, kReactorRK64_HIP>;
 template class ReactorHIP<PP_CHEM_BDF_ADAPTIVE, kReactorBDFAdaptive_HIP>;
 template class ReactorHIP<PP_CHEM_BDF_ADAPTIVE_MIXED, kReactorBDFBatched_HIP>;
+
+// Odr-use of each registration flag: the names are in the factory table from
+// static initialization on (explicit instantiation alone does not define the
+// implicit constructors that would otherwise use them).
+inline const bool kReactorHIPRegistered =
+  ReactorBase::Register<ReactorHIP<PP_CHEM_BDF, kReactorBDF_HIP>>::registered &&
+  ReactorBase::Register<ReactorHIP<PP_CHEM_RK64, kReactorRK64_HIP>>::registered &&
+  ReactorBase::Register<ReactorHIP<PP_CHEM_BDF_ADAPTIVE, kReactorBDFAdaptive_HIP>>::registered &&
+  ReactorBase::Register<ReactorHIP<PP_CHEM_BDF_ADAPTIVE_MIXED, kReactorBDFBatched_HIP>>::registered;
+
+#ifdef PP_CHEM_REFERENCE_NAMES
+inline constexpr char kReactorBDF_ref[] = "ReactorBDF";
+inline constexpr char kReactorRK64_ref[] = "ReactorRK64";
+template class ReactorHIP<PP_CHEM_BDF, kReactorBDF_ref>;
+template class ReactorHIP<PP_CHEM_RK64, kReactorRK64_ref>;
+inline const bool kReactorHIPReferenceNames =
+  ReactorBase::Register<ReactorHIP<PP_CHEM_BDF, kReactorBDF_ref>>::registered &&
+  ReactorBase::Register<ReactorHIP<PP_CHEM_RK64, kReactorRK64_ref>>::registered;
+#endif
 
 } // namespace pele::physics::reactions

@@ -827,7 +827,9 @@ static int rk64_core(sys_t* s, const ppo_rk64_params* p, double* y, double dt_re
  * solve_ivp(method="BDF"): modified Newton (<= 4 iterations), Jacobian
  * refreshed only after a Newton failure, RMS error norm with
  * scale = atol + rtol |y|, step factor limits [0.2, 10], order change after
- * order+1 equal steps. */
+ * order+1 equal steps; the structure follows SciPy's scipy/integrate/_ivp/bdf.py (BSD-3-Clause,
+ * Copyright (c) 2001-2002 Enthought, Inc. 2003-2024, SciPy Developers; the
+ * notice and license are in THIRD_PARTY_NOTICES.md at the repository root) */
 #define AMAXORD 5
 static const double NDF_KAPPA[6] = {0, -0.1850, -1.0 / 9.0, -0.0823, -0.0415, 0};
 

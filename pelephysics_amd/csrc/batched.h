@@ -10,6 +10,9 @@
 // cell it stores the result and takes the next cell from the queue (atomic
 // counter over a cost-ordered cell list).
 //
+// The algorithm follows SciPy's scipy/integrate/_ivp/bdf.py (BSD-3-Clause,
+// Copyright (c) 2001-2002 Enthought, Inc. 2003-2024, SciPy Developers; the
+// notice and license are in THIRD_PARTY_NOTICES.md at the repository root).
 // The step/order/Newton logic is the same NDF family as adaptive_cell
 // (reactors.h); the Newton matrix I - cJ is equilibrated with the error
 // weights, S^-1 (I - cJ) S, and factored in LU_T (float: FP32 LU with FP64

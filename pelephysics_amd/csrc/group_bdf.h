@@ -20,12 +20,12 @@
 // * Jacobian (rare: about one per cell and call): the same reaction split,
 //   accumulated with FP64 atomics into a per-group matrix in the device
 //   workspace, then each lane takes its rows (molecular-weight scaling of
-//   ReactorBDF.cpp:67-82) and keeps them in the workspace in LU_T for the LU
-//   refreshes;
-// * LU: row-per-lane Gaussian elimination with partial pivoting; the pivot is
-//   a group arg-max, the pivot row reaches the group through lane
-//   permutations, and rows never move (each row remembers the step at which it
-//   was the pivot).
+//   ReactorBDF.cpp:67-82) and keeps them in the workspace in LU_T for the
+//   Newton-matrix refreshes;
+// * Newton matrix: inverted in LU_T by row-per-lane Gauss-Jordan exchange
+//   steps with partial pivoting (the pivot a group arg-max, the pivot row
+//   through LDS, rows never move), so that each Newton iteration's solve is
+//   one dot product per row: no sequential triangular sweeps.
 //
 // Every group reduction returns the bitwise-identical value in all lanes of
 // the group (pairs are always added in commuted order on both partners), so
@@ -39,6 +39,9 @@
 // The step / order / Newton logic is that of batched_kernel (batched.h), the
 // NDF family of SciPy's BDF integrator; every loop over a cell's work is
 // bounded (queue length, RHS budget, the 200-iteration temperature inversion).
+// The algorithm follows SciPy's scipy/integrate/_ivp/bdf.py (BSD-3-Clause,
+// Copyright (c) 2001-2002 Enthought, Inc. 2003-2024, SciPy Developers; the
+// notice and license are in THIRD_PARTY_NOTICES.md at the repository root).
 #pragma once
 #include "chem_table.h"
 #include "reactors.h"
@@ -112,18 +115,68 @@ struct GroupChem {
   static constexpr int RPL = (NR + G - 1) / G;  // reaction slots per lane
   static constexpr int TL = NS % G, TS = NS / G;  // lane and slot of the temperature equation
 
-  // per-group LDS: RHS exchange, Jacobian species data, a vector for the
-  // LU's column scaling, the pivot rows of the LU
+  // per-group LDS: RHS exchange, Jacobian species data, the pivot row of
+  // the Newton-matrix inversion and two vectors for its column scaling and
+  // solves
   struct Shared {
     double c[NS], a[NS], b[NS], q[NR];
     double g[NS], h[NS], wd[NS];
-    double v[n];
-    int piv[n];
+    double v[n], w[n], prow[n];
   };
+
+  // Per-lane constants of the lane's species, set once per kernel: when a
+  // lane holds one species (SPL == 1) its NASA7 coefficients and range
+  // midpoint (no table loads in the hot loop); for every slot, imw_k times
+  // h_k/RT and cp_k/R (or e_k/RT, cv_k/R) at the bounds 90 K and 4000 K of
+  // the temperature inversion.
+  static constexpr int NC = SPL == 1 ? 1 : 0;
+  struct Cache {
+    double lo[NC > 0 ? NC : 1][7], hi[NC > 0 ? NC : 1][7], tmid[NC > 0 ? NC : 1];
+    double hb[2][SPL], cb[2][SPL];
+  };
+  PPC_DI static void init_cache(Cache& cc, const int g, const bool enth) {
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+      const int k = (e * G + g < NS) ? e * G + g : 0;
+#pragma unroll
+      for (int i = 0; i < 7; ++i) {
+        cc.lo[e][i] = T::nasaLo_(7 * k + i);
+        cc.hi[e][i] = T::nasaHi_(7 * k + i);
+      }
+      cc.tmid[e] = T::tmid_(k);
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const double Tb = b == 0 ? 90.0 : 4000.0;
+#pragma unroll
+      for (int e = 0; e < SPL; ++e) {
+        const int k = e * G + g;
+        cc.hb[b][e] = cc.cb[b][e] = 0.0;
+        if (k < NS) {
+          double a[7];
+          TM::nasa(k, Tb, a);
+          const double hp = TM::h_poly(a, Tb, 1.0 / Tb), cp = TM::cp_poly(a, Tb);
+          cc.hb[b][e] = T::imw_(k) * (enth ? hp : hp - 1.0);
+          cc.cb[b][e] = T::imw_(k) * (enth ? cp : cp - 1.0);
+        }
+      }
+    }
+  }
+  // NASA7 coefficients of slot e (species k) for temperature Tv
+  PPC_DI static void nasa(const Cache& cc, const int e, const int k, const double Tv, double* a) {
+    if constexpr (NC > 0) {
+      const bool lo = Tv < cc.tmid[0];
+#pragma unroll
+      for (int i = 0; i < 7; ++i) a[i] = lo ? cc.lo[0][i] : cc.hi[0][i];
+    } else {
+      TM::nasa(k, Tv, a);
+    }
+  }
 
   // mixture h (erg/g) and cp (erg/g/K), or e and cv, at Tv (Fuego.H CKHBMS/CKCPBS)
   template <bool ENTH>
-  PPC_DI static void mix(const int g, const double (&Y)[SPL], const double Tv, double& hm, double& cm) {
+  PPC_DI static void mix(const Cache& cc, const int g, const double (&Y)[SPL], const double Tv, double& hm,
+                         double& cm) {
     const double invT = 1.0 / Tv;
     double hs = 0.0, cs = 0.0;
 #pragma unroll
@@ -131,7 +184,7 @@ struct GroupChem {
       const int k = e * G + g;
       if (k < NS) {
         double a[7];
-        TM::nasa(k, Tv, a);
+        nasa(cc, e, k, Tv, a);
         double hp = TM::h_poly(a, Tv, invT), cp = TM::cp_poly(a, Tv);
         if (!ENTH) {
           hp -= 1.0;
@@ -146,22 +199,43 @@ struct GroupChem {
     cm = Gr::sum(cs) * RU;
   }
 
+  // mixture h (or e) at bound b (0: 90 K, 1: 4000 K) from the cached species
+  // values; with cp too only when the caller extrapolates
+  PPC_DI static double mix_bound(const Cache& cc, const int b, const double (&Y)[SPL], const bool want_c,
+                                 double& cm) {
+    double hs = 0.0, cs = 0.0;
+#pragma unroll
+    for (int e = 0; e < SPL; ++e) {
+      hs += Y[e] * cc.hb[b][e];
+      cs += Y[e] * cc.cb[b][e];
+    }
+    cm = want_c ? Gr::sum(cs) * RU : 0.0;
+    return Gr::sum(hs) * (RU * (b == 0 ? 90.0 : 4000.0));
+  }
+
   // GET_T_GIVEN_HY / GET_T_GIVEN_EY (Mechanisms/drm19/mechanism.H:2789-2890)
   // with group sums: Newton on T, |dT| clamped to 100 K, stop at |dT| < 1e-6,
-  // at most 200 iterations, linear extrapolation outside [90, 4000] K.
+  // at most 200 iterations, linear extrapolation outside [90, 4000] K.  (The
+  // cache was built for the same energy form.)
   template <bool ENTH>
-  PPC_DI static double T_from_energy(const int g, const double hv, const double (&Y)[SPL], const double tguess) {
+  PPC_DI static double T_from_energy(const Cache& cc, const int g, const double hv, const double (&Y)[SPL],
+                                     const double tguess) {
     const double tmin = 90.0, tmax = 4000.0;
-    double hmin, cmin, hmax, cmax;
-    mix<ENTH>(g, Y, tmin, hmin, cmin);
-    mix<ENTH>(g, Y, tmax, hmax, cmax);
-    if (hv < hmin) return tmin - (hmin - hv) / cmin;
-    if (hv > hmax) return tmax - (hmax - hv) / cmax;
+    double cmin, cmax;
+    const double hmin = mix_bound(cc, 0, Y, false, cmin), hmax = mix_bound(cc, 1, Y, false, cmax);
+    if (hv < hmin) {
+      mix_bound(cc, 0, Y, true, cmin);
+      return tmin - (hmin - hv) / cmin;
+    }
+    if (hv > hmax) {
+      mix_bound(cc, 1, Y, true, cmax);
+      return tmax - (hmax - hv) / cmax;
+    }
     double t1 = tguess;
     if (t1 < tmin || t1 > tmax) t1 = tmin + (tmax - tmin) / (hmax - hmin) * (hv - hmin);
     for (int i = 0; i < 200; ++i) {
       double h1, c1;
-      mix<ENTH>(g, Y, t1, h1, c1);
+      mix<ENTH>(cc, g, Y, t1, h1, c1);
       double dt = (hv - h1) / c1;
       if (dt > 100.0) {
         dt = 100.0;
@@ -202,7 +276,7 @@ struct GroupChem {
 
   // fKernelSpec (Source/Reactions/ReactorUtils.H:358-415) for the group's
   // cell: y (this lane's equation slots), f = ydot of the same slots.
-  PPC_DI static void rhs(Shared& sh, const int g, const int reactor_type, const double dt_save,
+  PPC_DI static void rhs(Shared& sh, const Cache& cc, const int g, const int reactor_type, const double dt_save,
                          const double (&y)[EPL], const double (&src)[SPL], const double rhoe_init,
                          const double rhoesrc_ext, double (&f)[EPL]) {
     double rsum = 0.0;
@@ -217,7 +291,7 @@ struct GroupChem {
     const double nrg = (rhoe_init + rhoesrc_ext * dt_save) * rinv;
     double Tv = Gr::bcast(y[TS], TL);
     const bool eform = reactor_type == E_REACTOR;
-    Tv = eform ? T_from_energy<false>(g, nrg, Y, Tv) : T_from_energy<true>(g, nrg, Y, Tv);
+    Tv = eform ? T_from_energy<false>(cc, g, nrg, Y, Tv) : T_from_energy<true>(cc, g, nrg, Y, Tv);
     const double invT = 1.0 / Tv, logT = log(Tv), RT = RU * Tv;
     double ek[SPL], csum = 0.0, msum = 0.0;
 #pragma unroll
@@ -226,7 +300,7 @@ struct GroupChem {
       ek[e] = 0.0;
       if (k < NS) {
         double a[7];
-        TM::nasa(k, Tv, a);
+        nasa(cc, e, k, Tv, a);
         const double hp = TM::h_poly(a, Tv, invT), cp = TM::cp_poly(a, Tv);
         const double imw = T::imw_(k);
         ek[e] = (eform ? hp - 1.0 : hp) * RT * imw;  // CKUMS / CKHMS
@@ -270,7 +344,8 @@ struct GroupChem {
   // (column-major, aJacobian / DWDOT layout), then row i = e*G + g written
   // to Jrow(e, j) in JT.
   template <int CONSP, typename JT, class JR>
-  PPC_DI static void jacobian(Shared& sh, const int g, double* Jw, const double (&y)[EPL], JR Jrow) {
+  PPC_DI static void jacobian(Shared& sh, const Cache& cc, const int g, double* Jw, const double (&y)[EPL],
+                              JR Jrow) {
     const double Tv = Gr::bcast(y[TS], TL);
     const double invT = 1.0 / Tv, invT2 = invT * invT, logT = log(Tv);
     const double refC = 101325.0 / 8.31446 * invT, refCinv = 1.0 / refC;
@@ -282,7 +357,7 @@ struct GroupChem {
       cR[e] = dcR[e] = eh[e] = sc[e] = 0.0;
       if (k < NS) {
         double a[7];
-        TM::nasa(k, Tv, a);
+        nasa(cc, e, k, Tv, a);
         const double hRT = TM::h_poly(a, Tv, invT), cpR = TM::cp_poly(a, Tv);
         sc[e] = y[e] * T::imw_(k) * 1.0e6;  // CKYTCR + DWDOT (x1e6)
         sh.c[k] = sc[e];
@@ -373,22 +448,25 @@ struct GroupChem {
   }
 };
 
-// --------------------------------------------------- group LU of rows --
-// Lane g holds rows i = e*G + g (e < EPL, i < n) of an n x n matrix in
-// a[e][0..n-1].  On return row i holds its multipliers in the columns before
-// the step kstep[e] at which it was the pivot and its U row from there on,
-// dinv[e] = 1 / U(i, kstep); piv[k] (group LDS) = the pivot row of step k.
+// ------------------------------------------- group Newton-matrix inverse --
+// Gauss-Jordan exchange steps on the row-distributed n x n matrix a (lane g
+// holds rows i = e*G + g, e < EPL) with partial pivoting: step k pivots on the
+// largest |a(i, k)| among the rows not yet used (the choice LU with partial
+// pivoting makes), lowest row on ties; rows never move.  The pivot row goes
+// through the group's LDS (prow).  Exchange on (r, k): a(r, k) <- 1 / a(r, k),
+// a(r, j) <- -a(r, j) / a(r, k), a(i, k) <- a(i, k) / a(r, k) and
+// a(i, j) <- a(i, j) - a(i, k) a(r, j) / a(r, k) for the other rows.  After the
+// n steps (kstep[e] = the step at which row i was the pivot) A x = b is
+// solved by x_{kstep(i)} = sum_j a(i, j) b_{p_j}, p_j the pivot row of step j:
+// one dot product per row instead of two sequential triangular sweeps.
 template <int n, int G, int EPL, typename LT>
-PPC_DI void group_lu_factor(LT (&a)[EPL][n], int (&kstep)[EPL], LT (&dinv)[EPL], int* piv, const int g) {
+PPC_DI void group_inv_factor(LT (&a)[EPL][n], int (&kstep)[EPL], double* prow, const int g) {
   using Gr = Grp<G>;
 #pragma unroll
-  for (int e = 0; e < EPL; ++e) {
-    kstep[e] = (e * G + g < n) ? -1 : n;
-    dinv[e] = LT(0);
-  }
+  for (int e = 0; e < EPL; ++e) kstep[e] = (e * G + g < n) ? -1 : n;
+  group_sync();
 #pragma unroll
   for (int k = 0; k < n; ++k) {
-    // pivot: largest |a(i, k)| over the rows not yet used, lowest row on ties
     unsigned long long key = 0;
 #pragma unroll
     for (int e = 0; e < EPL; ++e) {
@@ -402,71 +480,60 @@ PPC_DI void group_lu_factor(LT (&a)[EPL][n], int (&kstep)[EPL], LT (&dinv)[EPL],
     key = Gr::umax(key);
     const int p = (int)(0xffffffffu - (unsigned)(key & 0xffffffffu));
     const int pl = p % G, ps = p / G;
-    auto sel = [&](int col) {
-      LT v = a[0][col];
+    if (g == pl) {
 #pragma unroll
-      for (int e = 1; e < EPL; ++e) v = (ps == e) ? a[e][col] : v;
-      return v;
-    };
-    const LT apk = Gr::bcast(sel(k), pl);
-    const LT inv = LT(1) / apk;
+      for (int j = 0; j < n; ++j) {
+        LT v = a[0][j];
 #pragma unroll
-    for (int e = 0; e < EPL; ++e)
-      if (e * G + g == p) {
-        kstep[e] = k;
-        dinv[e] = inv;
+        for (int e = 1; e < EPL; ++e) v = (ps == e) ? a[e][j] : v;
+        prow[j] = (double)v;
       }
-    if (g == 0) piv[k] = p;
-    LT l[EPL];
+    }
+    group_sync();
+    const LT prs = LT(1) / (LT)prow[k];
 #pragma unroll
     for (int e = 0; e < EPL; ++e) {
-      // rows not yet used are eliminated; the others keep their values (l = 0)
-      const bool el = kstep[e] < 0;
-      l[e] = el ? a[e][k] * inv : LT(0);
-      a[e][k] = el ? l[e] : a[e][k];
-    }
+      const bool isp = e * G + g == p;
+      kstep[e] = isp ? k : kstep[e];
+      const LT f = a[e][k] * prs;
 #pragma unroll
-    for (int j = k + 1; j < n; ++j) {
-      const LT apj = Gr::bcast(sel(j), pl);
-#pragma unroll
-      for (int e = 0; e < EPL; ++e) a[e][j] -= l[e] * apj;
+      for (int j = 0; j < n; ++j) {
+        if (j == k) continue;
+        const LT r = (LT)prow[j];
+        a[e][j] = isp ? -r * prs : a[e][j] - f * r;
+      }
+      a[e][k] = isp ? prs : f;
     }
+    group_sync();
   }
-  group_sync();
 }
 
-// Solves with the factors of group_lu_factor: b (this lane's rows, in row
-// order) in, x (this lane's equation slots) out.
+// x (this lane's equation slots) = A^-1 b (this lane's rows) from the
+// exchanged matrix of group_inv_factor; v and w are n-vectors of the group's
+// LDS.
 template <int n, int G, int EPL, typename LT>
-PPC_DI void group_lu_solve(const LT (&a)[EPL][n], const int (&kstep)[EPL], const LT (&dinv)[EPL], const int* piv,
-                           const int g, double (&b)[EPL], double (&x)[EPL]) {
-  using Gr = Grp<G>;
-  auto sel = [&](const double (&v)[EPL], int ps) {
-    double r = v[0];
+PPC_DI void group_inv_solve(const LT (&a)[EPL][n], const int (&kstep)[EPL], double* v, double* w, const int g,
+                            const double (&b)[EPL], double (&x)[EPL]) {
 #pragma unroll
-    for (int e = 1; e < EPL; ++e) r = (ps == e) ? v[e] : r;
-    return r;
-  };
+  for (int e = 0; e < EPL; ++e)
+    if (e * G + g < n) v[kstep[e]] = b[e];  // v_j = b_{p_j}
+  group_sync();
+  double acc[EPL];
 #pragma unroll
-  for (int k = 0; k < n; ++k) {
-    const int p = piv[k];
-    const double bp = Gr::bcast(sel(b, p / G), p % G);
+  for (int e = 0; e < EPL; ++e) acc[e] = 0.0;
 #pragma unroll
-    for (int e = 0; e < EPL; ++e) b[e] = (kstep[e] > k) ? b[e] - (double)a[e][k] * bp : b[e];
+  for (int j = 0; j < n; ++j) {
+    const double vj = v[j];
+#pragma unroll
+    for (int e = 0; e < EPL; ++e) acc[e] += (double)a[e][j] * vj;
   }
-  double bd[EPL];
 #pragma unroll
-  for (int e = 0; e < EPL; ++e) x[e] = 0.0;
+  for (int e = 0; e < EPL; ++e)
+    if (e * G + g < n) w[kstep[e]] = acc[e];  // x_{kstep(i)}
+  group_sync();
 #pragma unroll
-  for (int k = n - 1; k >= 0; --k) {
-    const int p = piv[k];
-#pragma unroll
-    for (int e = 0; e < EPL; ++e) bd[e] = b[e] * (double)dinv[e];
-    const double xk = Gr::bcast(sel(bd, p / G), p % G);
-    x[k / G] = (g == k % G) ? xk : x[k / G];
-#pragma unroll
-    for (int e = 0; e < EPL; ++e) b[e] = (kstep[e] < k) ? b[e] - (double)a[e][k] * xk : b[e];
-  }
+  for (int e = 0; e < EPL; ++e) x[e] = (e * G + g < n) ? w[e * G + g] : 0.0;
+  group_sync();
 }
 
 // D[0..order][e] <- (R U)^T D[0..order][e] for a step-size ratio `factor`
@@ -530,6 +597,8 @@ group_bdf_kernel(ReactParamsDev P, IO1D io, double dt, QueueArgs q, CellStats st
   const int g = Gr::lane();
   const int gi = (int)__lane_id() / G;
   typename C::Shared& sh = shg[gi];
+  typename C::Cache cc;
+  C::init_cache(cc, g, P.reactor_type != E_REACTOR);
   const long long gid = (long long)blockIdx.x * NG + gi;
   double* Jw = P.ws + gid * group_ws_doubles<n, G, LU_T>();
   // this lane's Jacobian rows: element (e, j) at jrow[(e * n + j) * G]
@@ -550,9 +619,8 @@ group_bdf_kernel(ReactParamsDev P, IO1D io, double dt, QueueArgs q, CellStats st
   for (int e = 0; e < EPL; ++e) valid[e] = e * G + g < n;
   double D[AMAXORD + 3][EPL];
   double yv[EPL], f[EPL], d[EPL], ypred[EPL], psi[EPL], scl[EPL], lus[EPL], yjac[EPL], src[SPL];
-  LU_T LU[EPL][n];
+  LU_T LU[EPL][n];  // the exchanged (inverted) Newton matrix, this lane's rows
   int kstep[EPL];
-  LU_T dinv[EPL];
   double rhoe = 0, rhoesrc = 0;
   double t = 0, t_new = 0, h_abs = 0, c = 0, dy_norm_old = -1, h0 = 0;
   int order = 1, n_equal = 0, k_newton = 0, phase = PH_F0;
@@ -646,16 +714,16 @@ group_bdf_kernel(ReactParamsDev P, IO1D io, double dt, QueueArgs q, CellStats st
         LU[e][j] = valid[e] ? (LU_T)(si * m * sh.v[j]) : LU_T(0);
       }
     }
-    group_lu_factor<n, G, EPL, LU_T>(LU, kstep, dinv, sh.piv, g);
+    group_inv_factor<n, G, EPL, LU_T>(LU, kstep, sh.prow, g);
     nlu++;
     lu_valid = true;
   };
 
   auto jacobian = [&](const double (&yj)[EPL]) {
     if (P.reactor_type == H_REACTOR)
-      C::template jacobian<1, LU_T>(sh, g, Jw, yj, Jrow);
+      C::template jacobian<1, LU_T>(sh, cc, g, Jw, yj, Jrow);
     else
-      C::template jacobian<0, LU_T>(sh, g, Jw, yj, Jrow);
+      C::template jacobian<0, LU_T>(sh, cc, g, Jw, yj, Jrow);
     __threadfence();
     group_sync();
     njac++;
@@ -750,7 +818,7 @@ group_bdf_kernel(ReactParamsDev P, IO1D io, double dt, QueueArgs q, CellStats st
     }
     // ---- the one RHS evaluation of this trip
     const double t_eval = (phase == PH_F0) ? 0.0 : (phase == PH_F1 ? h0 : t_new);
-    C::rhs(sh, g, P.reactor_type, t_eval, yv, src, rhoe, rhoesrc, f);
+    C::rhs(sh, cc, g, P.reactor_type, t_eval, yv, src, rhoe, rhoesrc, f);
     nrhs++;
     bool ok = true;
     if (phase == PH_F0) {
@@ -805,7 +873,7 @@ group_bdf_kernel(ReactParamsDev P, IO1D io, double dt, QueueArgs q, CellStats st
         double b[EPL], dy[EPL];
 #pragma unroll
         for (int e = 0; e < EPL; ++e) b[e] = valid[e] ? (c * f[e] - psi[e] - d[e]) / lus[e] : 0.0;
-        group_lu_solve<n, G, EPL, LU_T>(LU, kstep, dinv, sh.piv, g, b, dy);
+        group_inv_solve<n, G, EPL, LU_T>(LU, kstep, sh.v, sh.w, g, b, dy);
 #pragma unroll
         for (int e = 0; e < EPL; ++e) dy[e] *= lus[e];
         const double dy_norm = rms(dy, scl);

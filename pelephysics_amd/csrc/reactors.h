@@ -273,6 +273,9 @@ PPC_DI int rk64_cell(const RK64ParamsDev& p, const int reactor_type, double* __r
 }
 
 // ------------------------------------------------ adaptive BDF (FP64) ---
+// The algorithm follows SciPy's scipy/integrate/_ivp/bdf.py (BSD-3-Clause,
+// Copyright (c) 2001-2002 Enthought, Inc. 2003-2024, SciPy Developers; the
+// notice and license are in THIRD_PARTY_NOTICES.md at the repository root).
 constexpr int AMAXORD = 5;
 
 PPC_DI double ndf_kappa(int k) {

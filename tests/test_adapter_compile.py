@@ -1,0 +1,44 @@
+"""The C++ adapter include/ReactorHIP.H compiled (g++) against the ReactorBase
+interface restated in tests/adapter/ReactorBase.H (Source/Reactions/
+ReactorBase.H:14-106, Source/Factory.H): every override must match the
+reference signatures, the four *_HIP names (and, with PP_CHEM_REFERENCE_NAMES,
+ReactorBDF / ReactorRK64) are in the factory table from static
+initialization, and init reports the reference's abort message for a bad
+reactor_type and an error for an unknown mechanism.  Runs without a GPU: the
+library rejects these configurations before any device call."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "pelephysics_amd", "libpele_chem_hip.so")
+
+
+@pytest.mark.parametrize("ref_names", [False, True])
+def test_adapter_compiles_registers_and_aborts(tmp_path, ref_names):
+    if not os.path.exists(LIB):
+        pytest.skip("libpele_chem_hip.so not built")
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    exe = tmp_path / "adapter"
+    cmd = [gxx, "-std=c++17", "-Wall", "-Werror", "-Wno-unused-variable",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "tests", "adapter"),
+           os.path.join(ROOT, "tests", "adapter", "adapter_main.cpp"), "-o", str(exe),
+           "-L", os.path.dirname(LIB), "-lpele_chem_hip", "-Wl,-rpath," + os.path.dirname(LIB)]
+    if ref_names:
+        cmd.insert(1, "-DPP_CHEM_REFERENCE_NAMES")
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr[-2000:]
+    names = {l.split()[1] for l in out.stdout.splitlines() if l.startswith("name ")}
+    want = {"ReactorBDF_HIP", "ReactorRK64_HIP", "ReactorBDFAdaptive_HIP", "ReactorBDFBatched_HIP"}
+    if ref_names:
+        want |= {"ReactorBDF", "ReactorRK64"}
+    assert names == want
+    assert "abort reactor_type: Wrong reactor type. Choose between 1 (e) or 2 (h)." in out.stdout
+    assert "abort mechanism: ReactorHIP: mechanism not compiled" in out.stdout
+    assert "abort create: In ReactorBase cannot find instance: ReactorNotThere" in out.stdout

@@ -126,3 +126,45 @@ def test_adjoint_autograd_wrapper(models):
     assert torch.allclose(dlnA.grad, ref, rtol=1e-12, atol=0.0)
     with pytest.raises(ValueError):
         BDFSensitivity.apply(dlnA.detach() + 1.0, y0, rX, R, NMACRO, DT)
+
+
+def test_adjoint_per_parameter_64_cells(models, oracles):
+    """Every sensitivity on its own: 64 of the 4096 cells of the adjoint
+    workload (every 64th), all 84 reactions and the three outputs; each g_j
+    within 1e-4 |fd_j| of its central difference wherever that difference
+    resolves it (1e-4 |fd_j| above fd_j's own noise: 4x the FMA / non-FMA
+    oracle spread of that quotient plus 64 ulp(o) / delta); at least half of
+    the (cell, output, reaction) triples are resolved this way."""
+    torch = _torch()
+    from pelephysics_amd.reactor import ReactorBase
+    m = models["drm19"]
+    O = oracles["drm19"]
+    names = [s.name for s in m.species]
+    kCO, kOH = names.index("CO"), names.index("OH")
+    st = W.subsample(W.drm19_fresh(m, 4096), np.arange(0, 4096, 64))
+    dev = torch.device("cuda:0")
+    R = ReactorBase.create("ReactorBDF")
+    R.init(2, st.ncells, mechanism="drm19")
+    y = torch.from_numpy(st.y_1d().reshape(st.ncells, -1)).to(dev)
+    rX = torch.from_numpy(st.rE.copy()).to(dev)
+    out, grad = R.adjoint_1d(y, rX, NMACRO, DT)
+    grad = grad.cpu().numpy()
+    o0, _ = _oracle_outputs(O, st, None, kCO, kOH)
+    assert np.allclose(out.cpu().numpy(), o0, rtol=1e-8, atol=1e-20)
+    nr = m.n_reactions
+    O2 = oracle.Mech(m, variant="nofma")
+    fd = np.zeros((st.ncells, nr, 3))
+    fd2 = np.zeros((st.ncells, nr, 3))
+    for j in range(nr):
+        e = np.zeros(nr)
+        e[j] = DELTA
+        fd[:, j] = (_oracle_outputs(O, st, e, kCO, kOH)[0] - _oracle_outputs(O, st, -e, kCO, kOH)[0]) / (2 * DELTA)
+        fd2[:, j] = (_oracle_outputs(O2, st, e, kCO, kOH)[0] - _oracle_outputs(O2, st, -e, kCO, kOH)[0]) / (2 * DELTA)
+    noise = 4 * np.abs(fd - fd2) + (64 * np.spacing(np.abs(o0)) / DELTA)[:, None, :]
+    resolved = 1e-4 * np.abs(fd) > noise
+    err = np.abs(grad - fd)
+    bad = resolved & (err > 1e-4 * np.abs(fd))
+    print(f"resolved {int(resolved.sum())} of {resolved.size} sensitivities, "
+          f"worst relative error {float((err / np.abs(fd))[resolved].max()):.3g}")
+    assert not bad.any(), np.argwhere(bad)[:10]
+    assert resolved.sum() >= resolved.size // 2
