@@ -86,9 +86,11 @@ typedef struct {
    * flat -> box (same per-cell operations, faster kernel build); 1 runs the
    * box kernel directly. */
   int batched_box_direct;
-  /* Batched forms: the kernel build held to 2 waves per SIMD (0 or 2, the
-   * default) or to 1 (more registers per lane, fewer spills; 1-D kernel only,
-   * so not with batched_box_direct = 1). */
+  /* The kernel build a launch uses.  Batched forms: held to 2 waves per SIMD
+   * (0 or 2, the default) or to 1 (more registers per lane, fewer spills;
+   * 1-D kernel only, so not with batched_box_direct = 1).  The plain adaptive
+   * BDF (PP_CHEM_BDF_ADAPTIVE): 0 (the compiler's choice), 2 or 4.  The
+   * parity integrators: 0. */
   int batched_waves_per_simd;
   /* Batched forms, cells per 64-lane wavefront: 0 (default) integrates one
    * cell per lane (64 per wave); 8, 4, 2 or 1 give each cell a group of 8, 16,

@@ -86,6 +86,11 @@ def lib(variant: str = "fma"):
         L.ppo_mech_create.restype = vp
         L.ppo_mech_create.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip,
                                       _dp, _dp, _ip, _dp, _dp, _dp, _dp]
+        L.ppo_mech_create_qss.restype = vp
+        L.ppo_mech_create_qss.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip,
+                                          _dp, _dp, _ip, _dp, _dp, _dp, _dp, _ip, C.c_int, _ip, _ip, _ip,
+                                          _ip, _dp]
+        L.ppo_qss_concentrations.argtypes = [vp, _dp, C.c_double, _dp]
         L.ppo_mech_destroy.argtypes = [vp]
         L.ppo_mech_set_dlnA.argtypes = [vp, C.c_void_p]
         for nm in ("ppo_cp_R", "ppo_h_RT", "ppo_g_RT"):
@@ -147,22 +152,25 @@ class Mech:
         self.model = model
         self.variant = variant
         ns, nr = model.n_species, model.n_reactions
-        self.ns, self.nr, self.n = ns, nr, ns + 1
+        nq = model.n_qss
+        nx = ns + nq  # QSS species follow the transported ones
+        self.ns, self.nr, self.n, self.nq = ns, nr, ns + 1, nq
         sp = model.species
         self.mw = np.array([s.mw for s in sp])
         self.imw = np.array([s.imw for s in sp])
-        tmid = np.array([s.tmid for s in sp])
-        lo = np.array([s.low for s in sp]).reshape(-1)
-        hi = np.array([s.high for s in sp]).reshape(-1)
+        allsp = model.all_species
+        tmid = np.array([s.tmid for s in allsp])
+        lo = np.array([s.low for s in allsp]).reshape(-1)
+        hi = np.array([s.high for s in allsp]).reshape(-1)
         kind = np.zeros(nr, np.int32)
         rev = np.zeros(nr, np.int32)
         arr = np.zeros((nr, 3))
         arrl = np.zeros((nr, 3))
         troen = np.zeros(nr, np.int32)
         troe = np.zeros((nr, 4))
-        eff = np.zeros((nr, ns))
-        nur = np.zeros((nr, ns))
-        nup = np.zeros((nr, ns))
+        eff = np.zeros((nr, nx))
+        nur = np.zeros((nr, nx))
+        nup = np.zeros((nr, nx))
         for j, r in enumerate(model.reactions):
             kind[j] = {"elementary": 0, "three-body": 1, "falloff": 2}[r.kind]
             rev[j] = int(r.reversible)
@@ -173,7 +181,7 @@ class Mech:
                 troen[j] = len(r.troe)
                 troe[j, :len(r.troe)] = r.troe
             if r.has_third_body:
-                eff[j, :] = r.default_eff
+                eff[j, :ns] = r.default_eff  # QSS species are not collision partners
                 for k, e in r.efficiencies.items():
                     eff[j, k] = e
             for k, v in r.reactants.items():
@@ -181,11 +189,25 @@ class Mech:
             for k, v in r.products.items():
                 nup[j, k] = v
         self._keep = [tmid, lo, hi, kind, rev, arr, arrl, troen, troe, eff, nur, nup]
-        self.nu_net = nup - nur
-        self.h = lib(variant).ppo_mech_create(ns, nr, self.mw, self.imw, tmid, lo, hi, kind, rev,
-                                        arr.reshape(-1), arrl.reshape(-1), troen,
-                                        troe.reshape(-1), eff.reshape(-1), nur.reshape(-1),
-                                        nup.reshape(-1))
+        self.nu_net = (nup - nur)[:, :ns]
+        if nq == 0:
+            self.h = lib(variant).ppo_mech_create(ns, nr, self.mw, self.imw, tmid, lo, hi, kind, rev,
+                                                  arr.reshape(-1), arrl.reshape(-1), troen,
+                                                  troe.reshape(-1), eff.reshape(-1), nur.reshape(-1),
+                                                  nup.reshape(-1))
+        else:
+            terms = model.qss_terms()
+            fwd = np.array([int(r.fwd_removed) for r in model.reactions], np.int32)
+            tr = np.array([t[0] for t in terms], np.int32)
+            tc = np.array([t[1] for t in terms], np.int32)
+            tx = np.array([t[2] for t in terms], np.int32)
+            td = np.array([t[3] for t in terms], np.int32)
+            tf = np.array([t[4] for t in terms], np.float64)
+            self._keep += [fwd, tr, tc, tx, td, tf]
+            self.h = lib(variant).ppo_mech_create_qss(ns, nq, nr, self.mw, self.imw, tmid, lo, hi, kind, rev,
+                                                      arr.reshape(-1), arrl.reshape(-1), troen,
+                                                      troe.reshape(-1), eff.reshape(-1), nur.reshape(-1),
+                                                      nup.reshape(-1), fwd, len(terms), tr, tc, tx, td, tf)
 
     def __del__(self):
         if getattr(self, "h", None) and self.variant in _libs:
@@ -239,6 +261,13 @@ class Mech:
         o = np.zeros(self.nr)
         lib(self.variant).ppo_progress_rates(self.h, np.ascontiguousarray(sc, float), T, o)
         return o
+
+    def qss_concentrations(self, sc, T):
+        """Concentrations (mol/m^3) of the QSS species at sc (mol/m^3), T."""
+        x = np.zeros(max(self.nq, 1))
+        lib(self.variant).ppo_qss_concentrations(self.h, np.ascontiguousarray(sc, dtype=np.float64),
+                                                 float(T), x)
+        return x[:self.nq]
 
     def jacobian(self, sc, T, consP):
         o = np.zeros(self.n * self.n)

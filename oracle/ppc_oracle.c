@@ -18,6 +18,14 @@
 
 struct ppo_mech {
   int ns, nr;
+  /* QSS mechanisms: nq species in quasi-steady state, indexed ns..nx-1 in the
+   * stoichiometry, efficiency and thermo tables; fwd_off[j] marks reactions
+   * whose forward direction the QSS reduction removed; the balance terms are
+   * (row, col, reaction, direction, coefficient) with col -1 for production,
+   * col = row for consumption and another col for a coupling. */
+  int nq, nx, nt;
+  int *fwd_off, *t_row, *t_col, *t_rx, *t_dir;
+  double* t_coef;
   double *mw, *imw, *tmid, *lo, *hi; /* lo/hi: ns x 7 */
   int *kind, *rev, *troe_n;
   double *arr, *arr_low, *troe, *eff, *nur, *nup, *scale;
@@ -37,9 +45,24 @@ ppo_mech* ppo_mech_create(int ns, int nr, const double* mw, const double* imw,
                           const double* arr_low, const int* troe_n,
                           const double* troe, const double* eff,
                           const double* nu_reac, const double* nu_prod) {
+  return ppo_mech_create_qss(ns, 0, nr, mw, imw, tmid, nasa_lo, nasa_hi, kind, reversible, arr, arr_low,
+                             troe_n, troe, eff, nu_reac, nu_prod, NULL, 0, NULL, NULL, NULL, NULL, NULL);
+}
+
+ppo_mech* ppo_mech_create_qss(int ns, int nq, int nr, const double* mw, const double* imw,
+                              const double* tmid, const double* nasa_lo, const double* nasa_hi,
+                              const int* kind, const int* reversible, const double* arr,
+                              const double* arr_low, const int* troe_n, const double* troe,
+                              const double* eff, const double* nu_reac, const double* nu_prod,
+                              const int* fwd_off, int nt, const int* t_row, const int* t_col,
+                              const int* t_rx, const int* t_dir, const double* t_coef) {
   ppo_mech* m = (ppo_mech*)calloc(1, sizeof(ppo_mech));
+  const int nx = ns + nq;
   m->ns = ns;
+  m->nq = nq;
+  m->nx = nx;
   m->nr = nr;
+  m->nt = nt;
 #define DUP(dst, src, cnt, T)                       \
   do {                                              \
     m->dst = (T*)malloc(sizeof(T) * (size_t)(cnt)); \
@@ -47,18 +70,27 @@ ppo_mech* ppo_mech_create(int ns, int nr, const double* mw, const double* imw,
   } while (0)
   DUP(mw, mw, ns, double);
   DUP(imw, imw, ns, double);
-  DUP(tmid, tmid, ns, double);
-  DUP(lo, nasa_lo, ns * 7, double);
-  DUP(hi, nasa_hi, ns * 7, double);
+  DUP(tmid, tmid, nx, double);
+  DUP(lo, nasa_lo, nx * 7, double);
+  DUP(hi, nasa_hi, nx * 7, double);
   DUP(kind, kind, nr, int);
   DUP(rev, reversible, nr, int);
   DUP(troe_n, troe_n, nr, int);
   DUP(arr, arr, nr * 3, double);
   DUP(arr_low, arr_low, nr * 3, double);
   DUP(troe, troe, nr * 4, double);
-  DUP(eff, eff, (size_t)nr * ns, double);
-  DUP(nur, nu_reac, (size_t)nr * ns, double);
-  DUP(nup, nu_prod, (size_t)nr * ns, double);
+  DUP(eff, eff, (size_t)nr * nx, double);
+  DUP(nur, nu_reac, (size_t)nr * nx, double);
+  DUP(nup, nu_prod, (size_t)nr * nx, double);
+  m->fwd_off = (int*)calloc(nr, sizeof(int));
+  if (fwd_off) memcpy(m->fwd_off, fwd_off, sizeof(int) * nr);
+  if (nt > 0) {
+    DUP(t_row, t_row, nt, int);
+    DUP(t_col, t_col, nt, int);
+    DUP(t_rx, t_rx, nt, int);
+    DUP(t_dir, t_dir, nt, int);
+    DUP(t_coef, t_coef, nt, double);
+  }
 #undef DUP
   m->scale = (double*)malloc(sizeof(double) * nr);
   for (int j = 0; j < nr; ++j) m->scale[j] = 1.0;
@@ -74,8 +106,8 @@ ppo_mech* ppo_mech_create(int ns, int nr, const double* mw, const double* imw,
   m->delta = (double*)calloc(nr, sizeof(double));
   for (int j = 0; j < nr; ++j) {
     double d = 0.0;
-    for (int k = 0; k < ns; ++k) {
-      double a = m->nur[(size_t)j * ns + k], b = m->nup[(size_t)j * ns + k];
+    for (int k = 0; k < nx; ++k) {
+      double a = m->nur[(size_t)j * nx + k], b = m->nup[(size_t)j * nx + k];
       d += a - b;
       if (a != 0.0) {
         m->ir[j * MAXSP + m->nr_r[j]] = k;
@@ -102,6 +134,7 @@ void ppo_mech_destroy(ppo_mech* m) {
   free(m->troe); free(m->eff); free(m->nur); free(m->nup); free(m->scale);
   free(m->nr_r); free(m->nr_p); free(m->nr_n); free(m->ir); free(m->ip);
   free(m->in); free(m->vr); free(m->vp); free(m->vn); free(m->delta);
+  free(m->fwd_off); free(m->t_row); free(m->t_col); free(m->t_rx); free(m->t_dir); free(m->t_coef);
   free(m);
 }
 
@@ -145,6 +178,18 @@ void ppo_g_RT(const ppo_mech* m, double T, double* out) {
   s_R(m, T, sb);
   for (int k = 0; k < m->ns; ++k) out[k] -= sb[k];
   if (sb != s) free(sb);
+}
+
+/* g/RT of every species, QSS species included (gibbs + gibbs_qss) */
+static void g_all(const ppo_mech* m, double T, double* out) {
+  for (int k = 0; k < m->nx; ++k) {
+    const double* a = nasa(m, k, T);
+    const double h = a[0] + a[1] / 2.0 * T + a[2] / 3.0 * T * T + a[3] / 4.0 * T * T * T +
+                     a[4] / 5.0 * T * T * T * T + a[5] / T;
+    const double sr = a[0] * log(T) + a[1] * T + a[2] / 2.0 * T * T + a[3] / 3.0 * T * T * T +
+                      a[4] / 4.0 * T * T * T * T + a[6];
+    out[k] = h - sr;
+  }
 }
 
 static void dcpR_dT(const ppo_mech* m, double T, double* out) {
@@ -289,7 +334,7 @@ static void rates(const ppo_mech* m, const double* sc, double T, double* wdot, d
     double corr = 1.0;
     if (m->kind[j] >= 1) {
       double alpha = 0.0;
-      for (int k = 0; k < ns; ++k) alpha += m->eff[(size_t)j * ns + k] * sc[k];
+      for (int k = 0; k < ns; ++k) alpha += m->eff[(size_t)j * m->nx + k] * sc[k];
       (void)mixture;
       if (m->kind[j] == 1) {
         corr = alpha;
@@ -316,17 +361,270 @@ static void rates(const ppo_mech* m, const double* sc, double T, double* wdot, d
   if (g != gbuf) free(g);
 }
 
+/* ------------------------------------------------- QSS mechanisms -- */
+/* The QSS concentrations x solve the regularised balances
+ *   (D_q - eps) x_q - sum_c (C_qc + eps) x_c = P_q + eps
+ * (the terms of pelephysics_amd/mechgen/model.py Mechanism.qss_terms, which
+ * restate Support/ceptr/ceptr/qssa_converter.py; eps as comp_sc_qss), where
+ * every D, C, P is a sum of reaction rates with the QSS concentration
+ * factored out (comp_qss_coeff).  The reference eliminates the system in the
+ * order its generator chose; here it is an LU factorisation with partial
+ * pivoting of the same matrix. */
+#define QSS_EPS 1e-12
+
+static int lu_factor(int n, double* a, int* piv);
+static void lu_solve(int n, const double* a, const int* piv, double* b);
+
+/* rate constants of reaction j with the dlnA scale: kf, kr = kf / Kc (0 when
+ * irreversible), the third-body / falloff factor corr and d ln(corr)/d alpha */
+static void rate_parts(const ppo_mech* m, int j, const double* sc, double T, const double* g, double* kf_o,
+                       double* kr_o, double* corr_o, double* dlnc_o) {
+  const double invT = 1.0 / T, logT = log(T);
+  const double refC = 101325.0 / 8.31446 * invT;
+  const double* a = m->arr + 3 * j;
+  const double kf = a[0] * exp(a[1] * logT - a[2] * invT);
+  double corr = 1.0, dlnc = 0.0;
+  if (m->kind[j] >= 1) {
+    double alpha = 0.0;
+    for (int k = 0; k < m->ns; ++k) alpha += m->eff[(size_t)j * m->nx + k] * sc[k];
+    if (m->kind[j] == 1) {
+      corr = alpha;
+      dlnc = 1.0 / alpha;
+    } else {
+      const double* al = m->arr_low + 3 * j;
+      const double k0 = al[0] * exp(al[1] * logT - al[2] * invT);
+      const double Pr = alpha * k0 / kf;
+      corr = Pr / (1.0 + Pr) * falloff_F(m, j, T, Pr);
+      double dlogFdlogPr = 0.0;
+      if (m->troe_n[j] > 0) {
+        const double* t = m->troe + 4 * j;
+        double Fcent = (1.0 - t[0]) * exp(-T / t[1]) + t[0] * exp(-T / t[2]);
+        if (m->troe_n[j] == 4) Fcent += exp(-t[3] * invT);
+        const double logFcent = log10(Fcent), logPr = log10(Pr);
+        const double tc = -0.4 - 0.67 * logFcent, tn = 0.75 - 1.27 * logFcent;
+        const double den = 1.0 / (tn - 0.14 * (tc + logPr));
+        const double tPr = (tc + logPr) * den;
+        const double troe = 1.0 / (1.0 + tPr * tPr);
+        dlogFdlogPr = -tn * 2.0 * logFcent * troe * troe * tPr * den * den;
+      }
+      dlnc = (1.0 / (1.0 + Pr) + dlogFdlogPr) / alpha;
+    }
+  }
+  double kr = 0.0;
+  if (m->rev[j]) {
+    double dg = 0.0;
+    for (int i = 0; i < m->nr_r[j]; ++i) dg += m->vr[j * MAXSP + i] * g[m->ir[j * MAXSP + i]];
+    for (int i = 0; i < m->nr_p[j]; ++i) dg -= m->vp[j * MAXSP + i] * g[m->ip[j * MAXSP + i]];
+    kr = kf / (exp(dg) * pow(refC, -m->delta[j]));
+  }
+  *kf_o = kf * m->scale[j];
+  *kr_o = kr * m->scale[j];
+  *corr_o = corr;
+  *dlnc_o = dlnc;
+}
+
+/* rate of one direction of reaction j with the QSS concentrations set to 1
+ * (sc1 holds the transported concentrations and ones): qf_co / qr_co */
+static double qss_co(const ppo_mech* m, int j, int dir, const double* sc1, double kf, double kr, double corr) {
+  if (dir == 0)
+    return m->fwd_off[j] ? 0.0 : corr * kf * prod_list(m->ir + j * MAXSP, m->vr + j * MAXSP, m->nr_r[j], sc1);
+  return corr * kr * prod_list(m->ip + j * MAXSP, m->vp + j * MAXSP, m->nr_p[j], sc1);
+}
+
+/* x (nq) and the LU factors of the balance matrix (A nq*nq, piv nq) */
+static void qss_solve(const ppo_mech* m, const double* sc, double T, const double* g, double* x, double* A,
+                      int* piv) {
+  const int ns = m->ns, nq = m->nq;
+  double* sc1 = (double*)malloc(sizeof(double) * m->nx);
+  for (int k = 0; k < m->nx; ++k) sc1[k] = k < ns ? sc[k] : 1.0;
+  for (int i = 0; i < nq * nq; ++i) A[i] = 0.0;
+  for (int q = 0; q < nq; ++q) {
+    A[q * nq + q] = -QSS_EPS;
+    x[q] = QSS_EPS;
+  }
+  for (int t = 0; t < m->nt; ++t)
+    if (m->t_col[t] >= 0 && m->t_col[t] != m->t_row[t]) A[m->t_row[t] * nq + m->t_col[t]] = -QSS_EPS;
+  for (int t = 0; t < m->nt; ++t) {
+    double kf, kr, corr, dlnc;
+    rate_parts(m, m->t_rx[t], sc, T, g, &kf, &kr, &corr, &dlnc);
+    const double v = m->t_coef[t] * qss_co(m, m->t_rx[t], m->t_dir[t], sc1, kf, kr, corr);
+    const int r = m->t_row[t], c = m->t_col[t];
+    if (c < 0)
+      x[r] += v;
+    else if (c == r)
+      A[r * nq + r] += v;
+    else
+      A[r * nq + c] -= v;
+  }
+  lu_factor(nq, A, piv);
+  lu_solve(nq, A, piv, x);
+  free(sc1);
+}
+
+static void rates_qss(const ppo_mech* m, const double* sc, double T, double* wdot, double* qout) {
+  const int ns = m->ns, nq = m->nq, nx = m->nx;
+  double* g = (double*)malloc(sizeof(double) * (2 * nx + nq * nq));
+  double* sce = g + nx;
+  double* A = sce + nx;
+  int* piv = (int*)malloc(sizeof(int) * nq);
+  g_all(m, T, g);
+  for (int k = 0; k < ns; ++k) sce[k] = sc[k];
+  qss_solve(m, sc, T, g, sce + ns, A, piv);
+  if (wdot) for (int k = 0; k < ns; ++k) wdot[k] = 0.0;
+  for (int j = 0; j < m->nr; ++j) {
+    double kf, kr, corr, dlnc;
+    rate_parts(m, j, sc, T, g, &kf, &kr, &corr, &dlnc);
+    double q = m->fwd_off[j] ? 0.0 : corr * kf * prod_list(m->ir + j * MAXSP, m->vr + j * MAXSP, m->nr_r[j], sce);
+    if (m->rev[j]) q -= corr * kr * prod_list(m->ip + j * MAXSP, m->vp + j * MAXSP, m->nr_p[j], sce);
+    if (qout) qout[j] = q;
+    if (wdot)
+      for (int i = 0; i < m->nr_n[j]; ++i) {
+        const int k = m->in[j * MAXSP + i];
+        if (k < ns) wdot[k] += m->vn[j * MAXSP + i] * q;
+      }
+  }
+  free(piv);
+  free(g);
+}
+
+void ppo_qss_concentrations(const ppo_mech* m, const double* sc, double T, double* x) {
+  const int nq = m->nq;
+  double* g = (double*)malloc(sizeof(double) * (m->nx + nq * nq));
+  int* piv = (int*)malloc(sizeof(int) * (nq > 0 ? nq : 1));
+  g_all(m, T, g);
+  if (nq > 0) qss_solve(m, sc, T, g, x, g + m->nx, piv);
+  free(piv);
+  free(g);
+}
+
+/* The species block is the exact derivative of the QSS-reduced production
+ * rates: the direct part at fixed x plus (d wdot / d x) (d x / d c) with
+ * d x / d c = -A^-1 d(balance)/d c (the reference's symbolic Jacobian,
+ * Support/ceptr/ceptr/jacobian.py ajac_symbolic, differentiates the same
+ * expressions; third-body sums count every species, whatever consP).  The
+ * temperature column is the forward difference with dT = 1e-2 K of the
+ * production rates, and the temperature row follows from both with consP
+ * (jacobian.py:386-470, Mechanisms/CH4_lean_qss/mechanism.H:9013-9080). */
+static void jacobian_qss(const ppo_mech* m, const double* sc, double T, int consP, double* J) {
+  const int ns = m->ns, nq = m->nq, nx = m->nx, n = ns + 1;
+  double* buf = (double*)calloc((size_t)(4 * nx + nq * nq + 2 * ns * nq + 6 * ns), sizeof(double));
+  double *g = buf, *sce = g + nx, *sc1 = sce + nx, *A = sc1 + nx, *Fx = A + nq * nq, *G = Fx + ns * nq;
+  double *w0 = G + nq * ns, *w1 = w0 + ns, *cR = w1 + ns, *dcR = cR + ns, *eh = dcR + ns, *hrt = eh + ns;
+  double* col = hrt + ns;
+  int* piv = (int*)malloc(sizeof(int) * nq);
+  memset(J, 0, sizeof(double) * n * n);
+  g_all(m, T, g);
+  for (int k = 0; k < ns; ++k) sce[k] = sc1[k] = sc[k];
+  for (int q = 0; q < nq; ++q) sc1[ns + q] = 1.0;
+  qss_solve(m, sc, T, g, sce + ns, A, piv);
+  const double* x = sce + ns;
+  /* direct part (columns of transported species) and d wdot / d x */
+  for (int j = 0; j < m->nr; ++j) {
+    double kf, kr, corr, dlnc;
+    rate_parts(m, j, sc, T, g, &kf, &kr, &corr, &dlnc);
+    const int* ir = m->ir + j * MAXSP;
+    const int* ip = m->ip + j * MAXSP;
+    const double* vr = m->vr + j * MAXSP;
+    const double* vp = m->vp + j * MAXSP;
+    const double ff = m->fwd_off[j] ? 0.0 : 1.0;
+    double q = ff * kf * prod_list(ir, vr, m->nr_r[j], sce);
+    if (m->rev[j]) q -= kr * prod_list(ip, vp, m->nr_p[j], sce);
+    q *= corr;
+    for (int k = 0; k < nx; ++k) {
+      double d = ff * kf * dprod(ir, vr, m->nr_r[j], sce, k);
+      if (m->rev[j]) d -= kr * dprod(ip, vp, m->nr_p[j], sce, k);
+      d *= corr;
+      if (m->kind[j] >= 1 && k < ns) d += m->eff[(size_t)j * nx + k] * dlnc * q;
+      if (d == 0.0) continue;
+      for (int i = 0; i < m->nr_n[j]; ++i) {
+        const int r = m->in[j * MAXSP + i];
+        if (r >= ns) continue;
+        if (k < ns)
+          J[k * n + r] += m->vn[j * MAXSP + i] * d;
+        else
+          Fx[r * nq + (k - ns)] += m->vn[j * MAXSP + i] * d;
+      }
+    }
+  }
+  /* d(balance)/d c: each term's rate with its weight in the balance */
+  for (int t = 0; t < m->nt; ++t) {
+    const int j = m->t_rx[t], dir = m->t_dir[t], r = m->t_row[t], c = m->t_col[t];
+    const double cf = m->t_coef[t];
+    if (dir == 0 && m->fwd_off[j]) continue;
+    double kf, kr, corr, dlnc;
+    rate_parts(m, j, sc, T, g, &kf, &kr, &corr, &dlnc);
+    const double wgt = c < 0 ? -cf : (c == r ? cf * x[r] : -cf * x[c]);
+    const int* il = dir == 0 ? m->ir + j * MAXSP : m->ip + j * MAXSP;
+    const double* vl = dir == 0 ? m->vr + j * MAXSP : m->vp + j * MAXSP;
+    const int cnt = dir == 0 ? m->nr_r[j] : m->nr_p[j];
+    const double k = dir == 0 ? kf : kr;
+    const double qco = corr * k * prod_list(il, vl, cnt, sc1);
+    for (int s = 0; s < ns; ++s) {
+      double d = corr * k * dprod(il, vl, cnt, sc1, s);
+      if (m->kind[j] >= 1) d += m->eff[(size_t)j * nx + s] * dlnc * qco;
+      G[r * ns + s] += wgt * d;
+    }
+  }
+  /* J[:, k] += Fx (-A^-1 G[:, k]) */
+  for (int k = 0; k < ns; ++k) {
+    for (int q = 0; q < nq; ++q) col[q] = -G[q * ns + k];
+    lu_solve(nq, A, piv, col);
+    for (int i = 0; i < ns; ++i) {
+      double d = 0.0;
+      for (int q = 0; q < nq; ++q) d += Fx[i * nq + q] * col[q];
+      J[k * n + i] += d;
+    }
+  }
+  /* temperature column by forward difference, then the temperature row */
+  const double pertT = 1e-2;
+  rates_qss(m, sc, T, w0, NULL);
+  rates_qss(m, sc, T + pertT, w1, NULL);
+  for (int k = 0; k < ns; ++k) J[ns * n + k] = (w1[k] - w0[k]) / pertT;
+  const double invT = 1.0 / T;
+  ppo_cp_R(m, T, cR);
+  if (consP != 1) for (int k = 0; k < ns; ++k) cR[k] -= 1.0;
+  dcpR_dT(m, T, dcR);
+  ppo_h_RT(m, T, hrt);
+  for (int k = 0; k < ns; ++k) eh[k] = (consP == 1) ? hrt[k] : hrt[k] - 1.0;
+  double cmix = 0.0, ehmix = 0.0, dcmixdT = 0.0, dehmixdT = 0.0;
+  for (int k = 0; k < ns; ++k) {
+    cmix += cR[k] * sc[k];
+    dcmixdT += dcR[k] * sc[k];
+    ehmix += eh[k] * w0[k];
+    dehmixdT += invT * (cR[k] - eh[k]) * w0[k] + eh[k] * J[ns * n + k];
+  }
+  const double cmixinv = 1.0 / cmix, tmp1 = ehmix * cmixinv, tmp3 = cmixinv * T, tmp2 = tmp1 * tmp3;
+  for (int k = 0; k < ns; ++k) {
+    double d = 0.0;
+    for (int mm = 0; mm < ns; ++mm) d += eh[mm] * J[k * n + mm];
+    J[k * n + ns] = tmp2 * cR[k] - tmp3 * d;
+  }
+  J[n * n - 1] = -tmp1 + tmp2 * dcmixdT - tmp3 * dehmixdT;
+  free(piv);
+  free(buf);
+}
+
 void ppo_production_rate(const ppo_mech* m, const double* sc, double T, double* wdot) {
-  rates(m, sc, T, wdot, NULL);
+  if (m->nq > 0)
+    rates_qss(m, sc, T, wdot, NULL);
+  else
+    rates(m, sc, T, wdot, NULL);
 }
 void ppo_progress_rates(const ppo_mech* m, const double* sc, double T, double* q) {
-  rates(m, sc, T, NULL, q);
+  if (m->nq > 0)
+    rates_qss(m, sc, T, NULL, q);
+  else
+    rates(m, sc, T, NULL, q);
 }
 
 /* aJacobian semantics (Mechanisms/LiDryer/mechanism.H:3905-5138): exact
  * derivatives, except that with consP == 1 the third-body/falloff terms use
  * d(alpha)/dc_k = eff_k - 1 (the mixture-concentration derivative dropped). */
 void ppo_jacobian(const ppo_mech* m, const double* sc, double T, int consP, double* J) {
+  if (m->nq > 0) {
+    jacobian_qss(m, sc, T, consP, J);
+    return;
+  }
   const int ns = m->ns, n = ns + 1;
   const double invT = 1.0 / T, invT2 = invT * invT, logT = log(T);
   const double refC = 101325.0 / 8.31446 * invT;
@@ -349,7 +647,7 @@ void ppo_jacobian(const ppo_mech* m, const double* sc, double T, int consP, doub
     double alpha = 0.0, Pr = 0.0, fPr = 1.0, F = 1.0, dlogFdlogPr = 0.0, dlogFdT = 0.0,
            dlogfPrdT = 0.0;
     if (m->kind[j] >= 1) {
-      for (int k = 0; k < ns; ++k) alpha += m->eff[(size_t)j * ns + k] * sc[k];
+      for (int k = 0; k < ns; ++k) alpha += m->eff[(size_t)j * m->nx + k] * sc[k];
     }
     if (m->kind[j] == 2) {
       const double* al = m->arr_low + 3 * j;
@@ -422,7 +720,7 @@ void ppo_jacobian(const ppo_mech* m, const double* sc, double T, int consP, doub
       double d = k_f * dprod(ir, vr, m->nr_r[j], sc, k);
       if (m->rev[j]) d -= k_r * dprod(ip, vp, m->nr_p[j], sc, k);
       if (has_fac) {
-        double e = m->eff[(size_t)j * ns + k];
+        double e = m->eff[(size_t)j * m->nx + k];
         d += (consP == 1 ? e - 1.0 : e) * fac;
       }
       if (d != 0.0)

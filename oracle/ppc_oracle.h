@@ -40,7 +40,22 @@ ppo_mech* ppo_mech_create(int ns, int nr, const double* mw, const double* imw,
                           const double* arr_low, const int* troe_n,
                           const double* troe, const double* eff,
                           const double* nu_reac, const double* nu_prod);
+/* The same for a QSS mechanism: nq species in quasi-steady state follow the
+ * ns transported ones (tmid, nasa_lo/hi over ns + nq species; eff, nu_reac,
+ * nu_prod nr x (ns + nq), QSS efficiencies 0); fwd_off[j] = 1 where the QSS
+ * reduction removed the forward direction; nt balance terms (t_row, t_col
+ * (-1 production, t_row consumption, else coupling), t_rx, t_dir (0 forward,
+ * 1 reverse), t_coef). */
+ppo_mech* ppo_mech_create_qss(int ns, int nq, int nr, const double* mw, const double* imw,
+                              const double* tmid, const double* nasa_lo, const double* nasa_hi,
+                              const int* kind, const int* reversible, const double* arr,
+                              const double* arr_low, const int* troe_n, const double* troe,
+                              const double* eff, const double* nu_reac, const double* nu_prod,
+                              const int* fwd_off, int nt, const int* t_row, const int* t_col,
+                              const int* t_rx, const int* t_dir, const double* t_coef);
 void ppo_mech_destroy(ppo_mech* m);
+/* QSS concentrations (mol/m^3) of a QSS mechanism at sc (mol/m^3), T */
+void ppo_qss_concentrations(const ppo_mech* m, const double* sc, double T, double* x);
 /* Multiplies k_f and k_r of reaction j by exp(dlnA[j]) (NULL resets). */
 void ppo_mech_set_dlnA(ppo_mech* m, const double* dlnA);
 

@@ -29,7 +29,11 @@ keeps the fastest:
 
 Each candidate runs `calls` calls on the same cells and arrays; its time is
 the kernel time of the last one (the queue in the cost order of the call
-before).  A candidate is eligible only if every cell integrates and its
+before).  By default every call restarts from the same initial state
+(`--protocol replay`); with `--protocol chained` each call continues from the
+state the previous one left, as a flow solver's chemistry steps do (the
+benchmark's protocol), and the candidates are compared on the last call of
+that chain.  A candidate is eligible only if every cell integrates and its
 result agrees with the first candidate's to |x - x_ref| <= 1e-6 |x_ref| +
 1e-10 on mass fractions and T (the launch shapes do not change any cell's
 arithmetic, so a disagreement means a faulty build).  Results go to a JSON table
@@ -52,6 +56,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE_ONLY = ("dodecane_lu", "heptane_lu_88sk")
 BATCHED = ("ReactorBDFAdaptiveMixed", "ReactorBDFBatchedFP64")
 OCCUPANCY = ((2, 8), (2, 6), (2, 4), (1, 4), (1, 3))  # (waves per SIMD, waves per CU)
+PLAIN_OCCUPANCY = (0, 2, 4)  # plain adaptive BDF builds: compiler choice, 2 or 4 waves per SIMD
 LU_SERVICE = ((16, 4), (64, 8), (64, 32))
 GROUPS = (8, 4, 2, 1)  # cells per wave of the group kernel
 GROUP_WAVES = (0, 4)  # persistent waves per CU of the group kernel (0: every resident wave)
@@ -85,8 +90,13 @@ def candidates(mech: str, integrator: str, form: str = "1d") -> list:
     have = set(mechanisms())
     impls = [k for k in (mech, mech + "_table") if k in have]
     if integrator not in BATCHED:
-        return [{"kernels": k, "batched_waves_per_simd": 0, "batched_waves_per_cu": 0, "batched_lu_batch": 0,
-                 "batched_lu_wait": 0, "batched_box_direct": 0} for k in impls]
+        # the plain adaptive BDF is also built held to 2 and 4 waves per SIMD
+        # for mechanisms whose cells fit in registers (NEQ <= 32: no workspace)
+        from .mechgen.__main__ import load
+        fits = load(mech).n_species + 1 <= 32
+        occs = PLAIN_OCCUPANCY if integrator == "ReactorBDFAdaptive" and fits else (0,)
+        return [{"kernels": k, "batched_waves_per_simd": o, "batched_waves_per_cu": 0, "batched_lu_batch": 0,
+                 "batched_lu_wait": 0, "batched_box_direct": 0} for k in impls for o in occs]
     routes = (0, 1) if form == "box" else (0,)
     lane = [{"kernels": k, "batched_waves_per_simd": o, "batched_waves_per_cu": w, "batched_lu_batch": b,
              "batched_lu_wait": t, "batched_box_direct": r, "batched_cells_per_wave": 0}
@@ -102,7 +112,7 @@ def candidates(mech: str, integrator: str, form: str = "1d") -> list:
 
 
 def _time(cand: dict, integrator: str, form: str, st, m, dt: float, rtol: float, atol: float,
-          calls: int):
+          calls: int, chained: bool = False):
     """(kernel ms of the last call, final state (ncells, N+1) as T and mass fractions)."""
     import torch
     from .reactor import ReactorBase
@@ -123,9 +133,10 @@ def _time(cand: dict, integrator: str, form: str, st, m, dt: float, rtol: float,
         rYs, rEs, FC = torch.zeros_like(rY0), torch.zeros_like(rE0), torch.zeros_like(T0)
         rY, T, rE = rY0.clone(), T0.clone(), rE0.clone()  # the same arrays each call: cost order
         for _ in range(calls):
-            rY.copy_(rY0)
-            T.copy_(T0)
-            rE.copy_(rE0)
+            if not chained:
+                rY.copy_(rY0)
+                T.copy_(T0)
+                rE.copy_(rE0)
             R.react(rY, rYs, T, rE, rEs, FC, dt)
             ms = R.stats()["kernel_ms"]
         out = np.concatenate([rY.reshape(ns, -1).T.cpu().numpy(), T.reshape(-1, 1).cpu().numpy()], axis=1)
@@ -136,8 +147,9 @@ def _time(cand: dict, integrator: str, form: str, st, m, dt: float, rtol: float,
         xs = torch.zeros(st.ncells, dtype=torch.float64, device=dev)
         y, x = y0.clone(), x0.clone()
         for _ in range(calls):
-            y.copy_(y0)
-            x.copy_(x0)
+            if not chained:
+                y.copy_(y0)
+                x.copy_(x0)
             R.react_1d(y, ys, x, xs, dt)
             ms = R.stats()["kernel_ms"]
         out = y.reshape(st.ncells, ns + 1).cpu().numpy()
@@ -153,7 +165,7 @@ def _agrees(a, b) -> bool:
 
 def tune(mech: str, integrator: str, form: str = "box", ncells: int = 262144, dt: float = 1e-5,
          rtol: float = 1e-10, atol: float = 1e-16, calls: int = 2, out: str | None = None,
-         log=print) -> dict:
+         log=print, chained: bool = False, groups: bool = True) -> dict:
     from .mechgen.__main__ import load
     from .throughput import workload
     st = workload(mech, ncells)
@@ -161,11 +173,13 @@ def tune(mech: str, integrator: str, form: str = "box", ncells: int = 262144, dt
     from .reactor import ReactorError
     results, ref = [], None
     for cand in candidates(mech, integrator, form):
+        if not groups and cand.get("batched_cells_per_wave"):
+            continue
         # every launch shape integrates each cell with the same arithmetic: a
         # candidate whose cells fail or whose result leaves the criterion
         # around the first candidate's is not eligible
         try:
-            ms, res = _time(cand, integrator, form, st, m, dt, rtol, atol, calls)
+            ms, res = _time(cand, integrator, form, st, m, dt, rtol, atol, calls, chained)
         except ReactorError as e:
             results.append(dict(cand, kernel_ms=None, rejected=str(e)))
             log(f"[autotune] {mech} {integrator} {form} {cand}: rejected ({e})")
@@ -180,7 +194,8 @@ def tune(mech: str, integrator: str, form: str = "box", ncells: int = 262144, dt
         raise RuntimeError(f"no candidate of {mech} {integrator} {form} integrated every cell")
     best = min(eligible, key=lambda r: r["kernel_ms"])
     entry = dict({k: best[k] for k in best if k != "kernel_ms"}, kernel_ms=best["kernel_ms"], cells=st.ncells,
-                 dt=dt, rtol=rtol, atol=atol, candidates=results)
+                 dt=dt, rtol=rtol, atol=atol, calls=calls, protocol="chained" if chained else "replay",
+                 candidates=results)
     a = arch()
     path = out or table_path(a)
     table = {}
@@ -205,9 +220,12 @@ def main(argv=None):
     ap.add_argument("--atol", type=float, default=1e-16)
     ap.add_argument("--calls", type=int, default=2)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--protocol", choices=["replay", "chained"], default="replay")
+    ap.add_argument("--no-groups", action="store_true", help="skip the group-kernel candidates")
     a = ap.parse_args(argv)
     e = tune(a.mech, a.integrator, a.form, a.cells, a.dt, a.rtol, a.atol, a.calls, a.out,
-             log=lambda *x: print(*x, file=sys.stderr, flush=True))
+             log=lambda *x: print(*x, file=sys.stderr, flush=True), chained=a.protocol == "chained",
+             groups=not a.no_groups)
     print(json.dumps({k: v for k, v in e.items() if k != "candidates"}))
 
 

@@ -521,7 +521,15 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
     return PP_CHEM_EINVAL;
   if (cfg->clean_init_massfrac != 0 && cfg->clean_init_massfrac != 1) return PP_CHEM_EINVAL;
   if (cfg->batched_box_direct != 0 && cfg->batched_box_direct != 1) return PP_CHEM_EINVAL;
-  if (cfg->batched_waves_per_simd < 0 || cfg->batched_waves_per_simd > 2) return PP_CHEM_EINVAL;
+  // the kernel build: batched forms 0, 1 or 2 waves per SIMD; the plain
+  // adaptive BDF 0 (the compiler's choice), or 2 or 4 for mechanisms whose
+  // cells fit in registers (no workspace); the parity integrators 0
+  const int occ = cfg->batched_waves_per_simd;
+  if (is_batched(cfg->mode) ? (occ < 0 || occ > 2)
+                            : (cfg->mode == PP_CHEM_BDF_ADAPTIVE
+                                   ? (occ != 0 && ((occ != 2 && occ != 4) || L->ws_rows(cfg->mode) > 0))
+                                   : occ != 0))
+    return PP_CHEM_EINVAL;
   if (cfg->batched_waves_per_simd == 1 && cfg->batched_box_direct == 1) return PP_CHEM_EINVAL;  // no such build
   if (cfg->batched_cells_per_wave != 0) {
     // a group kernel of this size must exist for the mechanism; box calls go through flat state

@@ -84,9 +84,15 @@ struct has_table<M, std::void_t<decltype(M::kTable)>> : std::integral_constant<b
 // Groups of G lanes per cell exist for the table-driven mechanisms with at most
 // three equations per lane (NEQ <= 3 G) and at least half the lanes holding
 // an equation (G <= 2 NEQ).
+template <class M, class = void>
+struct has_qss : std::false_type {};
+template <class M>
+struct has_qss<M, std::void_t<decltype(M::NQ)>> : std::integral_constant<bool, (M::NQ > 0)> {};
+
+// (QSS mechanisms run one lane per cell.)
 template <class M, int G>
 constexpr bool group_ok() {
-  return has_table<M>::value && M::NEQ <= 3 * G && G <= 2 * M::NEQ;
+  return has_table<M>::value && !has_qss<M>::value && M::NEQ <= 3 * G && G <= 2 * M::NEQ;
 }
 
 // Device workspace of one group: the n x n FP64 Jacobian accumulator and its

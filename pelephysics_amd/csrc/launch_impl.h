@@ -33,13 +33,22 @@ int lane_grid(const ReactParamsDev& P, int ncells, int block) {
   return (ncells + block - 1) / block;
 }
 
+// The plain adaptive BDF is built at the compiler's occupancy and, for
+// mechanisms without a workspace, also held to 2 and 4 waves per SIMD
+// (P.adapt.lane_occ); the parity integrators at the compiler's choice only.
 template <class M, int MODE>
 hipError_t launch_1d_mode(const ReactParamsDev& P, double* y, const double* ysrc, double* rX,
                           const double* rXsrc, double dt, int ncells, CellStats st, int block, hipStream_t s) {
   const int grid = lane_grid<M, MODE>(P, ncells, block);
   if (grid < 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((react_1d_kernel<M, MODE>), dim3(grid), dim3(block), 0, s, P, y, ysrc, rX, rXsrc, dt,
-                     ncells, st);
+  if constexpr (MODE == MODE_ADAPTIVE && !uses_ws<M>()) {
+    auto k = P.adapt.lane_occ == 4 ? react_1d_kernel_held<M, MODE, 4>
+                                   : (P.adapt.lane_occ == 2 ? react_1d_kernel_held<M, MODE, 2> : react_1d_kernel<M, MODE>);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, s, P, y, ysrc, rX, rXsrc, dt, ncells, st);
+  } else {
+    hipLaunchKernelGGL((react_1d_kernel<M, MODE>), dim3(grid), dim3(block), 0, s, P, y, ysrc, rX, rXsrc, dt,
+                       ncells, st);
+  }
   return hipGetLastError();
 }
 
@@ -48,7 +57,13 @@ hipError_t launch_box_mode(const ReactParamsDev& P, const BoxArgs& b, double dt,
                            int block, hipStream_t s) {
   const int grid = lane_grid<M, MODE>(P, ncells, block);
   if (grid < 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((react_box_kernel<M, MODE>), dim3(grid), dim3(block), 0, s, P, b, dt, ncells, st);
+  if constexpr (MODE == MODE_ADAPTIVE && !uses_ws<M>()) {
+    auto k = P.adapt.lane_occ == 4 ? react_box_kernel_held<M, MODE, 4>
+                                   : (P.adapt.lane_occ == 2 ? react_box_kernel_held<M, MODE, 2> : react_box_kernel<M, MODE>);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, s, P, b, dt, ncells, st);
+  } else {
+    hipLaunchKernelGGL((react_box_kernel<M, MODE>), dim3(grid), dim3(block), 0, s, P, b, dt, ncells, st);
+  }
   return hipGetLastError();
 }
 
@@ -181,7 +196,8 @@ template <class M>
 hipError_t launch_adjoint(const ReactParamsDev& P, const AdjointArgs& a, hipStream_t s) {
   using Tab = typename wave_tables<M>::type;
   // one equation per lane, at most two reactions per lane
-  if constexpr (!std::is_void<Tab>::value && M::NEQ <= 64 && M::NR <= 128) {
+  // (not QSS mechanisms: the adjoint differentiates plain mass-action rates)
+  if constexpr (!std::is_void<Tab>::value && !has_qss<M>::value && M::NEQ <= 64 && M::NR <= 128) {
     auto k = wave_adjoint_kernel<Tab>;
     const int grid = resident_blocks(k, (long long)a.ncells * 64);
     if (grid < 0) return hipErrorInvalidDevice;

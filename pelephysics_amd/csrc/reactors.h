@@ -669,11 +669,9 @@ PPC_DI void store_counters(const CellStats& st, int c, int nsteps, const int* cn
 // lane per cell; with a workspace (lane_ws.h) the grid is what the workspace
 // holds and lanes stride over the cells.
 template <class M, int MODE>
-__global__ void __launch_bounds__(64) react_1d_kernel(ReactParamsDev P, double* __restrict__ yv,
-                                                      const double* __restrict__ ysrc,
-                                                      double* __restrict__ rX,
-                                                      const double* __restrict__ rXsrc, double dt,
-                                                      int ncells, CellStats st) {
+PPC_DI void react_1d_cells(const ReactParamsDev& P, double* __restrict__ yv, const double* __restrict__ ysrc,
+                           double* __restrict__ rX, const double* __restrict__ rXsrc, double dt, int ncells,
+                           const CellStats& st) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const LaneWS w{P.ws, P.ws_lanes, g};
   constexpr int n = M::NEQ;
@@ -694,8 +692,7 @@ __global__ void __launch_bounds__(64) react_1d_kernel(ReactParamsDev P, double* 
 // initial T from energy, integration, rE += dt*rEsrc, final T from energy with
 // the integrated T as guess, FC = per-cell step count.
 template <class M, int MODE>
-__global__ void __launch_bounds__(64) react_box_kernel(ReactParamsDev P, BoxArgs b, double dt,
-                                                       int ncells, CellStats st) {
+PPC_DI void react_box_cells(const ReactParamsDev& P, const BoxArgs& b, double dt, int ncells, const CellStats& st) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const LaneWS w{P.ws, P.ws_lanes, g};
   constexpr int NS = M::NS, n = M::NEQ;
@@ -722,6 +719,37 @@ __global__ void __launch_bounds__(64) react_box_kernel(ReactParamsDev P, BoxArgs
     if (b.FC.p) b.FC(i, j, k, 0) = (double)r;
     store_counters(st, c, r, cnt);
   }
+}
+
+// The kernels at the compiler's choice of occupancy ...
+template <class M, int MODE>
+__global__ void __launch_bounds__(64) react_1d_kernel(ReactParamsDev P, double* __restrict__ yv,
+                                                      const double* __restrict__ ysrc,
+                                                      double* __restrict__ rX,
+                                                      const double* __restrict__ rXsrc, double dt,
+                                                      int ncells, CellStats st) {
+  react_1d_cells<M, MODE>(P, yv, ysrc, rX, rXsrc, dt, ncells, st);
+}
+
+template <class M, int MODE>
+__global__ void __launch_bounds__(64) react_box_kernel(ReactParamsDev P, BoxArgs b, double dt, int ncells,
+                                                       CellStats st) {
+  react_box_cells<M, MODE>(P, b, dt, ncells, st);
+}
+
+// ... and held to OCC waves per SIMD (2: 256 VGPRs, 4: 128 VGPRs per lane).
+template <class M, int MODE, int OCC>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
+react_1d_kernel_held(ReactParamsDev P, double* __restrict__ yv, const double* __restrict__ ysrc,
+                     double* __restrict__ rX, const double* __restrict__ rXsrc, double dt, int ncells,
+                     CellStats st) {
+  react_1d_cells<M, MODE>(P, yv, ysrc, rX, rXsrc, dt, ncells, st);
+}
+
+template <class M, int MODE, int OCC>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
+react_box_kernel_held(ReactParamsDev P, BoxArgs b, double dt, int ncells, CellStats st) {
+  react_box_cells<M, MODE>(P, b, dt, ncells, st);
 }
 
 }  // namespace ppc

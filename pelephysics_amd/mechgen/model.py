@@ -75,6 +75,9 @@ class Reaction:
     troe: Optional[List[float]] = None  # [A, T3, T1] or [A, T3, T1, T2]
     default_eff: float = 1.0
     efficiencies: Dict[int, float] = field(default_factory=dict)
+    # forward direction dropped by the QSS reduction (qssa.yaml
+    # forward_to_remove_idx): the reaction runs in reverse only
+    fwd_removed: bool = False
 
     @property
     def has_third_body(self) -> bool:
@@ -104,16 +107,95 @@ class Species:
     high: List[float]  # NASA7 coefficients a1..a7 for T >= tmid
 
 
+# Regularisation of the quasi-steady-state balances: every structural entry of
+# the QSS system (diagonal, coupling, right-hand side) carries it, as the
+# reference's generated comp_sc_qss does (epsilon = 1e-12,
+# Support/ceptr/ceptr/qssa_converter.py).
+QSS_EPS = 1e-12
+
+
 @dataclass
 class Mechanism:
+    """species: the transported species (the state); qss_species: species in
+    quasi-steady state, indexed n_species + q in reaction stoichiometry."""
+
     name: str
     elements: List[str]
     species: List[Species]
     reactions: List[Reaction]
+    qss_species: List[Species] = field(default_factory=list)
 
     @property
     def n_species(self) -> int:
         return len(self.species)
+
+    @property
+    def n_qss(self) -> int:
+        return len(self.qss_species)
+
+    @property
+    def all_species(self) -> List[Species]:
+        return self.species + self.qss_species
+
+    def qss_terms(self) -> List[Tuple[int, int, int, int, float]]:
+        """The quasi-steady-state balance of each QSS species q as terms
+        (q, col, reaction, direction, coefficient): direction 0 takes the
+        forward rate and 1 the reverse rate of the reaction with the QSS
+        concentration factored out (qf_co / qr_co); col = -1 adds
+        coefficient x rate to the production P_q, col = q adds it to the
+        consumption D_q, another col c to the coupling C_qc.  The balance is
+        (D_q - eps) x_q - sum_c (C_qc + eps) x_c = P_q + eps.
+
+        The rules restate Support/ceptr/ceptr/qssa_converter.py:960-1250: a
+        reaction with one QSS species contributes its consumption (rate x 1)
+        and production (stoichiometric coefficient x rate); with two, the
+        rate in the direction that turns one into the other couples them; a
+        reaction whose forward direction was removed and whose QSS species
+        are all reactants contributes only its reverse production, and an
+        irreversible reaction producing two QSS species contributes nothing."""
+        NS = self.n_species
+        terms = []
+        for q in range(self.n_qss):
+            s = NS + q
+            for j, r in enumerate(self.reactions):
+                inR, inP = s in r.reactants, s in r.products
+                if not (inR or inP):
+                    continue
+                if inR and inP:
+                    raise ValueError(f"QSS species on both sides of {r.equation!r}")
+                Q = sorted({k for k in list(r.reactants) + list(r.products) if k >= NS})
+                nuR, nuP = r.reactants.get(s, 0.0), r.products.get(s, 0.0)
+                if len(Q) == 1:
+                    if inR:
+                        terms.append((q, q, j, 0, 1.0))
+                        if r.reversible:
+                            terms.append((q, -1, j, 1, nuR))
+                    else:
+                        terms.append((q, -1, j, 0, nuP))
+                        if r.reversible:
+                            terms.append((q, q, j, 1, 1.0))
+                    continue
+                if len(Q) > 2:
+                    raise ValueError(f"more than two QSS species in {r.equation!r}")
+                o = Q[0] if Q[1] == s else Q[1]
+                if not any(k in r.products for k in Q) and r.fwd_removed:
+                    if r.reversible:
+                        terms.append((q, -1, j, 1, nuR))
+                elif inR:
+                    terms.append((q, q, j, 0, 1.0))
+                    if r.reversible:
+                        terms.append((q, o - NS, j, 1, nuR))
+                elif o in r.reactants:
+                    terms.append((q, o - NS, j, 0, nuP))
+                    if r.reversible:
+                        terms.append((q, q, j, 1, 1.0))
+        return terms
+
+    def qss_reactions(self) -> List[int]:
+        """Reactions (YAML order) with a QSS species on either side."""
+        NS = self.n_species
+        return [j for j, r in enumerate(self.reactions)
+                if any(k >= NS for k in list(r.reactants) + list(r.products))]
 
     @property
     def n_reactions(self) -> int:
@@ -138,8 +220,12 @@ class Mechanism:
             "species": [asdict(s) for s in self.species],
             "reactions": [],
         }
+        if self.qss_species:
+            d["qss_species"] = [asdict(s) for s in self.qss_species]
         for r in self.reactions:
             rd = asdict(r)
+            if not rd["fwd_removed"]:
+                del rd["fwd_removed"]
             # JSON object keys must be strings
             for key in ("reactants", "products", "efficiencies"):
                 rd[key] = {str(k): v for k, v in rd[key].items()}
@@ -158,7 +244,8 @@ class Mechanism:
             rd["rate"] = Arrhenius(**rd["rate"])
             rd["low"] = Arrhenius(**rd["low"]) if rd["low"] is not None else None
             reactions.append(Reaction(**rd))
-        return Mechanism(d["name"], d["elements"], species, reactions)
+        qss = [Species(**s) for s in d.get("qss_species", [])]
+        return Mechanism(d["name"], d["elements"], species, reactions, qss)
 
 
 # ---------------------------------------------------------------- YAML input
@@ -198,11 +285,29 @@ def _parse_side(text: str, names: Dict[str, int]) -> Tuple[Dict[int, float], boo
     return stoich, third, collider
 
 
-def _arrhenius(d: dict, order: float) -> Arrhenius:
-    """CGS-mol pre-exponential of a rate of the given concentration order -> SI-mol."""
-    A_cgs = float(d["A"])
-    A = A_cgs * 10.0 ** (-6.0 * (order - 1.0))
-    return Arrhenius(A=A, b=float(d.get("b", 0.0)), EaR=float(d.get("Ea", 0.0)) / RUC)
+# Units of a Cantera YAML file: (concentration unit in mol/m^3, activation
+# energy unit x K per Ea/R).  A file without a "units" block is in Cantera's
+# defaults (m, kmol, J/kmol), as the QSS converter writes qssa.yaml; energies
+# in joules convert to cal/mol with 4.184 J/cal before the division by Rc, as
+# the reference's generator does (Mechanisms/CH4_lean_qss/mechanism.H reaction
+# 0: Ea = 5.14632e6 J/kmol -> 618.956498939805 K).
+def _unit_system(units: dict):
+    length = units.get("length", "m")
+    quantity = units.get("quantity", "kmol")
+    energy = units.get("activation-energy", "J/kmol")
+    conc = {("cm", "mol"): 1e6, ("m", "kmol"): 1e3, ("m", "mol"): 1.0}.get((length, quantity))
+    rgas = {"cal/mol": RUC, "J/kmol": 4184.0 * RUC, "J/mol": 4.184 * RUC}.get(energy)
+    if conc is None or rgas is None:
+        raise ValueError(f"unsupported units {units}")
+    return conc, rgas
+
+
+def _arrhenius(d: dict, order: float, units=(1e6, RUC)) -> Arrhenius:
+    """Pre-exponential of a rate of the given concentration order in the
+    file's units (default CGS-mol, cal/mol) -> SI-mol, activation energy -> K."""
+    conc, rgas = units
+    A = float(d["A"]) * conc ** (-(order - 1.0))
+    return Arrhenius(A=A, b=float(d.get("b", 0.0)), EaR=float(d.get("Ea", 0.0)) / rgas)
 
 
 def load_yaml(path: str, name: Optional[str] = None) -> Mechanism:
@@ -210,14 +315,16 @@ def load_yaml(path: str, name: Optional[str] = None) -> Mechanism:
 
     with open(path) as f:
         doc = yaml.safe_load(f)
-    units = doc.get("units", {})
-    if units.get("length", "cm") != "cm" or units.get("quantity", "mol") != "mol":
-        raise ValueError("only cm/mol mechanisms are supported")
-    if units.get("activation-energy", "cal/mol") != "cal/mol":
-        raise ValueError("only cal/mol activation energies are supported")
+    units = _unit_system(doc.get("units", {}))
 
     phase = doc["phases"][0]
-    sp_names: List[str] = list(phase["species"])
+    # QSS mechanisms (qssa.yaml): transported species first, then the QSS
+    # species, each in phase order
+    qss_names = set(phase.get("qssa_species", []))
+    sp_names: List[str] = ([n for n in phase["species"] if n not in qss_names] +
+                           [n for n in phase["species"] if n in qss_names])
+    n_transported = len(sp_names) - len(qss_names)
+    removed = set(phase.get("forward_to_remove_idx", []))
     sp_by_name = {s["name"]: s for s in doc["species"]}
     names = {n: i for i, n in enumerate(sp_names)}
 
@@ -263,16 +370,18 @@ def load_yaml(path: str, name: Optional[str] = None) -> Mechanism:
         rtype = rd.get("type", "elementary")
         order = sum(reac.values())
         eff = {names[k]: float(v) for k, v in rd.get("efficiencies", {}).items() if k in names}
+        if any(k >= n_transported for k in eff):
+            raise ValueError(f"third-body efficiency of a QSS species in {eq!r}")
         default_eff = float(rd.get("default-efficiency", 1.0))
         if rtype == "elementary":
             if tb_l or tb_r or col_l:
                 raise ValueError(f"elementary reaction with third body: {eq!r}")
-            r = Reaction(idx, eq, reac, prod, rev, "elementary", _arrhenius(rd["rate-constant"], order))
+            r = Reaction(idx, eq, reac, prod, rev, "elementary", _arrhenius(rd["rate-constant"], order, units))
         elif rtype == "three-body":
             if not (tb_l and tb_r):
                 raise ValueError(f"explicit-collider three-body reactions unsupported: {eq!r}")
             r = Reaction(idx, eq, reac, prod, rev, "three-body",
-                         _arrhenius(rd["rate-constant"], order + 1.0),
+                         _arrhenius(rd["rate-constant"], order + 1.0, units),
                          default_eff=default_eff, efficiencies=eff)
         elif rtype == "falloff":
             if col_l != "M":
@@ -286,12 +395,23 @@ def load_yaml(path: str, name: Optional[str] = None) -> Mechanism:
             elif "SRI" in rd:
                 raise ValueError(f"SRI falloff unsupported: {eq!r}")
             r = Reaction(idx, eq, reac, prod, rev, "falloff",
-                         _arrhenius(rd["high-P-rate-constant"], order),
-                         low=_arrhenius(rd["low-P-rate-constant"], order + 1.0),
+                         _arrhenius(rd["high-P-rate-constant"], order, units),
+                         low=_arrhenius(rd["low-P-rate-constant"], order + 1.0, units),
                          troe=troe, default_eff=default_eff, efficiencies=eff)
         else:
             raise ValueError(f"unsupported reaction type {rtype!r}: {eq!r}")
+        r.fwd_removed = idx in removed
+        if r.fwd_removed and not rev:
+            raise ValueError(f"forward direction removed from irreversible {eq!r}")
+        # the QSS balance is linear in the QSS concentrations
+        if sum(v for k, v in reac.items() if k >= n_transported) > 1 and not r.fwd_removed:
+            raise ValueError(f"QSS species quadratic in the forward rate of {eq!r}")
+        if sum(v for k, v in prod.items() if k >= n_transported) > 1 and rev:
+            raise ValueError(f"QSS species quadratic in the reverse rate of {eq!r}")
         reactions.append(r)
 
     elements = [str(e) for e in phase["elements"]]
-    return Mechanism(name or phase.get("name", "gas"), elements, species, reactions)
+    m = Mechanism(name or phase.get("name", "gas"), elements, species[:n_transported], reactions,
+                  species[n_transported:])
+    m.qss_terms()  # validates the QSS structure
+    return m

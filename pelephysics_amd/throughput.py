@@ -27,7 +27,8 @@ from .mechgen.__main__ import load
 from .reactor import ReactorBase
 
 GENERATORS = {"LiDryer": W.lidryer_h2air, "dodecane_lu": W.dodecane_uneven,
-              "heptane_lu_88sk": W.heptane_ignition}
+              "heptane_lu_88sk": W.heptane_ignition, "dodecane_lu_qss": W.dodecane_uneven,
+              "CH4_lean_qss": W.premixed, "LuEthylene_qss": W.premixed}
 
 
 def workload(mech: str, ncells: int) -> W.CellState:

@@ -163,7 +163,9 @@ def subsample(st: CellState, idx: np.ndarray) -> CellState:
 
 # Fuel of each mechanism and its stoichiometric O2 per fuel molecule.
 FUELS = {"LiDryer": ("H2", 0.5), "drm19": ("CH4", 2.0), "dodecane_lu": ("NC12H26", 18.5),
-         "heptane_lu_88sk": ("NC7H16", 11.0)}
+         "heptane_lu_88sk": ("NC7H16", 11.0), "CH4_lean_qss": ("CH4", 2.0),
+         "dodecane_lu_qss": ("NC12H26", 18.5), "heptane_lu_qss": ("NC7H16", 11.0),
+         "LuEthylene_qss": ("C2H4", 3.0)}
 
 
 def _fuel_air_Y(m: Mechanism, phi: np.ndarray) -> np.ndarray:
@@ -214,7 +216,7 @@ def premixed(m: Mechanism, ncells: int, seed: int = SEED) -> CellState:
         return lidryer_h2air(m, ncells, seed)
     if m.name == "drm19":
         return drm19_fresh(m, ncells, seed)
-    if m.name == "dodecane_lu":
+    if m.name in ("dodecane_lu", "dodecane_lu_qss", "CH4_lean_qss", "LuEthylene_qss"):
         rng = np.random.Generator(np.random.PCG64(seed))
         T = rng.uniform(900.0, 1200.0, ncells)
         phi = rng.uniform(0.7, 1.3, ncells)

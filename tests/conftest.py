@@ -15,7 +15,8 @@ def pytest_configure(config):
 @pytest.fixture(scope="session")
 def models():
     from pelephysics_amd.mechgen.__main__ import load
-    return {m: load(m) for m in ("LiDryer", "drm19", "dodecane_lu", "heptane_lu_88sk")}
+    return {m: load(m) for m in ("LiDryer", "drm19", "dodecane_lu", "heptane_lu_88sk", "CH4_lean_qss",
+                                 "LuEthylene_qss", "dodecane_lu_qss")}
 
 
 @pytest.fixture(scope="session")

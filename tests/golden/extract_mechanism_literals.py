@@ -157,6 +157,11 @@ def extract(mech: str) -> dict:
     nsp = int(m.group(1))
     mw = [float(x.split("//")[0].strip().rstrip(",")) for x in m.group(2).strip().split("\n") if x.strip()]
     thermo = {fn: _thermo_values(text, fn, nsp) for fn in ("cp_R", "speciesEnthalpy", "gibbs")}
+    # QSS mechanisms: reactions also read the QSS species' g/RT and
+    # concentrations (g_RT_qss, sc_qss); "reactants" and "net" list the
+    # transported species only, as productionRate reports them
+    mq = re.search(r"amrex::Real g_RT_qss\[(\d+)\];", text)
+    nqss = int(mq.group(1)) if mq else 0
 
     reactions = []
     for idx, eq, flat in _production_blocks(text):
@@ -200,7 +205,7 @@ def extract(mech: str) -> dict:
         mq = re.search(r"qr = (.*?);", flat)
         qr = mq.group(1) if mq else "0.0"
         env = {"Corr": 1.0, "k_f": 1.0, "g_RT": [0.0] * nsp, "sc": [1.0] * nsp,
-               "refC": 2.0, "refCinv": 0.5}
+               "g_RT_qss": [0.0] * nqss, "sc_qss": [1.0] * nqss, "refC": 2.0, "refCinv": 0.5}
         # some irreversible reactions are emitted with a reverse rate times a
         # literal 0.0 (e.g. heptane_lu_88sk reaction 21)
         r["reversible"] = "g_RT" in qr and _eval(qr, env) != 0.0

@@ -26,6 +26,10 @@ def test_candidates():
     c = autotune.candidates("dodecane_lu", "ReactorBDFAdaptive")
     assert c == [{"kernels": "dodecane_lu", "batched_waves_per_simd": 0, "batched_waves_per_cu": 0,
                   "batched_lu_batch": 0, "batched_lu_wait": 0, "batched_box_direct": 0}]
+    # the plain adaptive BDF of a register-resident mechanism: three builds per implementation
+    c = autotune.candidates("drm19", "ReactorBDFAdaptive", "box")
+    assert {(x["kernels"], x["batched_waves_per_simd"]) for x in c} == {
+        (k, o) for k in ("drm19", "drm19_table") for o in autotune.PLAIN_OCCUPANCY}
 
 
 def test_lookup_reads_the_table(tmp_path, monkeypatch):
