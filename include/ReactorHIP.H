@@ -10,16 +10,22 @@
 //   "ReactorRK64_HIP"         -> PP_CHEM_RK64          (ReactorRK64 semantics)
 //   "ReactorBDFAdaptive_HIP"  -> PP_CHEM_BDF_ADAPTIVE  (adaptive FP64 BDF)
 //   "ReactorBDFBatched_HIP"   -> PP_CHEM_BDF_ADAPTIVE_MIXED (batched, FP32 LU)
+//   "ReactorCvode_HIP"        -> PP_CHEM_CVODE         (ReactorCvode's options and
+//                                defaults on the FP64 adaptive BDF; cvode.max_order,
+//                                cvode.max_substeps, ode.max_nls_iters and the
+//                                typical values of set_typ_vals_ode)
 // The ode.* ParmParse keys of ReactorBDF::init / ReactorRK64::init are read
 // with the same names; ode.rtol / ode.atol / ode.max_steps for the adaptive
 // forms.  The mechanism name comes from ode.mechanism (default: the string of
 // the compiled PelePhysics mechanism passed as PP_CHEM_MECHANISM).
 // With PP_CHEM_REFERENCE_NAMES defined (a build that leaves the reference's
-// ReactorBDF.cpp / ReactorRK64.cpp out of PelePhysics), "ReactorBDF" and
-// "ReactorRK64" are registered too, so an unchanged inputs file selects them.
+// ReactorBDF.cpp / ReactorRK64.cpp / ReactorCvode.cpp out of PelePhysics),
+// "ReactorBDF", "ReactorRK64" and "ReactorCvode" are registered too, so an
+// unchanged inputs file selects them.
 #pragma once
 
 #include <string>
+#include <vector>
 
 #include "ReactorBase.H"
 #include "pele_chem_hip.h"
@@ -68,6 +74,13 @@ public:
     pp.query("batched_cells_per_wave", m_cfg.batched_cells_per_wave);
     pp.query("batched_box_direct", m_cfg.batched_box_direct);
     pp.query("workspace_bytes", m_cfg.workspace_bytes);
+    if constexpr (MODE == PP_CHEM_CVODE) {
+      // ReactorCvode.cpp:20-31
+      pp.query("max_nls_iters", m_cfg.max_nls_iters);
+      amrex::ParmParse ppcv("cvode");
+      ppcv.query("max_order", m_cfg.cvode_max_order);
+      ppcv.query("max_substeps", m_cfg.max_steps);
+    }
     bool clean = false;
     pp.query("clean_init_massfrac", clean);  // ReactorBDF.cpp:127, ReactorRK64.cpp:18
     m_cfg.clean_init_massfrac = clean ? 1 : 0;
@@ -127,6 +140,7 @@ public:
 #ifdef AMREX_USE_GPU
     s = (void*)stream;
 #endif
+    sync_typ_vals();
     const int r = pp_chem_react_box(
       m_h, l, h, &a[0], &a[1], &a[2], &a[3], &a[4], &a[5], nullptr, &dt_react, &time, s);
     if (r < 0) {
@@ -153,6 +167,7 @@ public:
 #ifdef AMREX_USE_GPU
     s = (void*)stream;
 #endif
+    sync_typ_vals();
     // the reference's 1-D overload receives host pointers on GPU builds
     const int r = pp_chem_react_1d(
       m_h, rY_in, rYsrc_in, rX_in, rX_src_in, &dt_react, &time, ncells, 0, s);
@@ -228,6 +243,26 @@ private:
 #endif
   }
 
+  // ReactorCvode's absolute tolerances follow the typical values the base
+  // keeps (ReactorBase::set_typ_vals_ode is not virtual): handed over before
+  // a call whenever they changed
+  void sync_typ_vals()
+  {
+    if constexpr (MODE == PP_CHEM_CVODE) {
+      constexpr int n = NUM_SPECIES + 1;
+      std::vector<double> v(n);
+      for (int k = 0; k < n; ++k) {
+        v[k] = this->m_typ_vals[k];
+      }
+      if (v != m_typ_sent) {
+        if (pp_chem_set_typ_vals(m_h, v.data(), n) != 0) {
+          amrex::Abort(std::string("ReactorHIP: ") + pp_chem_last_error(m_h));
+        }
+        m_typ_sent = v;
+      }
+    }
+  }
+
   template <class T>
   static pp_array4 view(amrex::Array4<T> const& a)
   {
@@ -249,17 +284,20 @@ private:
   pp_chem_config m_cfg{};
   std::string m_mech;
   pp_chem* m_h = nullptr;
+  std::vector<double> m_typ_sent;
 };
 
 inline constexpr char kReactorBDF_HIP[] = "ReactorBDF_HIP";
 inline constexpr char kReactorRK64_HIP[] = "ReactorRK64_HIP";
 inline constexpr char kReactorBDFAdaptive_HIP[] = "ReactorBDFAdaptive_HIP";
 inline constexpr char kReactorBDFBatched_HIP[] = "ReactorBDFBatched_HIP";
+inline constexpr char kReactorCvode_HIP[] = "ReactorCvode_HIP";
 
 template class ReactorHIP<PP_CHEM_BDF, kReactorBDF_HIP>;
 template class ReactorHIP<PP_CHEM_RK64, kReactorRK64_HIP>;
 template class ReactorHIP<PP_CHEM_BDF_ADAPTIVE, kReactorBDFAdaptive_HIP>;
 template class ReactorHIP<PP_CHEM_BDF_ADAPTIVE_MIXED, kReactorBDFBatched_HIP>;
+template class ReactorHIP<PP_CHEM_CVODE, kReactorCvode_HIP>;
 
 // Odr-use of each registration flag: the names are in the factory table from
 // static initialization on (explicit instantiation alone does not define the
@@ -268,16 +306,20 @@ inline const bool kReactorHIPRegistered =
   ReactorBase::Register<ReactorHIP<PP_CHEM_BDF, kReactorBDF_HIP>>::registered &&
   ReactorBase::Register<ReactorHIP<PP_CHEM_RK64, kReactorRK64_HIP>>::registered &&
   ReactorBase::Register<ReactorHIP<PP_CHEM_BDF_ADAPTIVE, kReactorBDFAdaptive_HIP>>::registered &&
-  ReactorBase::Register<ReactorHIP<PP_CHEM_BDF_ADAPTIVE_MIXED, kReactorBDFBatched_HIP>>::registered;
+  ReactorBase::Register<ReactorHIP<PP_CHEM_BDF_ADAPTIVE_MIXED, kReactorBDFBatched_HIP>>::registered &&
+  ReactorBase::Register<ReactorHIP<PP_CHEM_CVODE, kReactorCvode_HIP>>::registered;
 
 #ifdef PP_CHEM_REFERENCE_NAMES
 inline constexpr char kReactorBDF_ref[] = "ReactorBDF";
 inline constexpr char kReactorRK64_ref[] = "ReactorRK64";
+inline constexpr char kReactorCvode_ref[] = "ReactorCvode";
 template class ReactorHIP<PP_CHEM_BDF, kReactorBDF_ref>;
 template class ReactorHIP<PP_CHEM_RK64, kReactorRK64_ref>;
+template class ReactorHIP<PP_CHEM_CVODE, kReactorCvode_ref>;
 inline const bool kReactorHIPReferenceNames =
   ReactorBase::Register<ReactorHIP<PP_CHEM_BDF, kReactorBDF_ref>>::registered &&
-  ReactorBase::Register<ReactorHIP<PP_CHEM_RK64, kReactorRK64_ref>>::registered;
+  ReactorBase::Register<ReactorHIP<PP_CHEM_RK64, kReactorRK64_ref>>::registered &&
+  ReactorBase::Register<ReactorHIP<PP_CHEM_CVODE, kReactorCvode_ref>>::registered;
 #endif
 
 } // namespace pele::physics::reactions

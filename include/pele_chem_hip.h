@@ -38,7 +38,14 @@ enum {
   PP_CHEM_BDF_ADAPTIVE = 2,        /* adaptive-order BDF with error control, one lane per cell, FP64 */
   PP_CHEM_BDF_ADAPTIVE_MIXED = 3,  /* batched adaptive BDF: persistent work queue, FP32 Newton-matrix
                                       LU, FP64 residuals, rates and error control */
-  PP_CHEM_BDF_BATCHED_FP64 = 4     /* batched adaptive BDF with an FP64 LU */
+  PP_CHEM_BDF_BATCHED_FP64 = 4,    /* batched adaptive BDF with an FP64 LU */
+  PP_CHEM_CVODE = 5                /* "ReactorCvode": the FP64 adaptive BDF (one lane per cell) with
+                                      ReactorCvode's options and defaults (rtol 1e-6, atol 1e-10 x the
+                                      typical values, order <= cvode.max_order = 2, ode.max_nls_iters
+                                      = 3 Newton iterations, Jacobian age <= 100 steps,
+                                      cvode.max_substeps = 10000); FC and the return value count RHS
+                                      evaluations (ReactorCvode.cpp:1365-1372).  Not SUNDIALS' CVODE:
+                                      the same option set on this library's NDF/BDF integrator. */
 };
 
 enum {
@@ -99,6 +106,12 @@ typedef struct {
    * a mechanism supports depend on its size; box calls go through flat
    * state). */
   int batched_cells_per_wave;
+  /* PP_CHEM_BDF_ADAPTIVE and PP_CHEM_CVODE (0 elsewhere): order cap 1..5
+   * (cvode.max_order; 0: 5), Newton iterations per step attempt
+   * (ode.max_nls_iters; 0: 4), and the Jacobian's largest age in steps before
+   * it is re-evaluated at the predicted state (CVodeSetJacEvalFrequency; 0:
+   * only when Newton fails to converge). */
+  int cvode_max_order, max_nls_iters, jac_max_age;
 } pp_chem_config;
 
 /* amrex::Array4<Real> layout: element (i,j,k,n) is at
@@ -118,6 +131,13 @@ typedef struct {
 } pp_chem_stats;
 
 typedef struct pp_chem pp_chem;
+
+/* ReactorBase::set_typ_vals_ode (ReactorBase.cpp:6-31, read by
+ * ReactorUtils.H:119-131): n = NUM_SPECIES + 1 typical values (species, then
+ * T); ReactorCvode then uses the absolute tolerances atol x typ_vals[k]
+ * (typ_vals[0] <= 0: the scalar atol again).  The other integrators ignore
+ * them, as the reference's ReactorBDF and ReactorRK64 do. */
+int pp_chem_set_typ_vals(pp_chem* h, const double* typ_vals, int n);
 
 /* Fill cfg with the reference defaults for the given mechanism and mode. */
 int pp_chem_config_init(pp_chem_config* cfg, const char* mechanism, int mode, int reactor_type);

@@ -34,7 +34,9 @@ class RK64Params(C.Structure):
 
 class AdaptParams(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double),
-                ("first_step", C.c_double), ("max_steps", C.c_int)]
+                ("first_step", C.c_double), ("max_steps", C.c_int),
+                ("max_order", C.c_int), ("newton_maxiter", C.c_int), ("jac_age", C.c_int),
+                ("atolv", C.POINTER(C.c_double))]
 
 
 class AdaptStats(C.Structure):
@@ -141,8 +143,16 @@ def rk64_params(**kw) -> RK64Params:
     return RK64Params(**d)
 
 
-def adapt_params(rtol=1e-8, atol=1e-10, first_step=0.0, max_steps=100000) -> AdaptParams:
-    return AdaptParams(rtol, atol, first_step, max_steps)
+def adapt_params(rtol=1e-8, atol=1e-10, first_step=0.0, max_steps=100000, max_order=0, newton_maxiter=0,
+                 jac_age=0, atolv=None) -> AdaptParams:
+    """Adaptive BDF parameters; max_order, newton_maxiter, jac_age and a
+    per-component atolv (length n) are the CVODE-style options (0 / None:
+    SciPy's behaviour)."""
+    p = AdaptParams(rtol, atol, first_step, max_steps, max_order, newton_maxiter, jac_age, None)
+    if atolv is not None:
+        p._atolv = np.ascontiguousarray(atolv, dtype=np.float64)
+        p.atolv = p._atolv.ctypes.data_as(C.POINTER(C.c_double))
+    return p
 
 
 class Mech:

@@ -1235,16 +1235,23 @@ static int adaptive_core(sys_t* s, const ppo_adapt_params* p, double* y0, double
   for (int i = 0; i <= AMAXORD; ++i) econst[i] = NDF_KAPPA[i] * gamma[i] + 1.0 / (i + 1);
   econst[AMAXORD + 1] = 0.0;
   const double newton_tol = fmax(10 * EPS / rtol, fmin(0.03, sqrt(rtol)));
+  /* CVODE-style options (ReactorCvode.cpp:206-224, ReactorUtils.H:119-131):
+   * order cap, Newton iterations per attempt, Jacobian age in steps, and
+   * absolute tolerances per component (atol x typical value); 0 / NULL keep
+   * SciPy's behaviour (order 5, 4 iterations, no age limit, scalar atol) */
+  const int maxord = (p->max_order > 0 && p->max_order < AMAXORD) ? p->max_order : AMAXORD;
+  const int nmax = p->newton_maxiter > 0 ? p->newton_maxiter : 4;
+#define ATOL(i) (p->atolv ? p->atolv[i] : atol)
   double* D = w->D;
   double t = 0.0;
-  int nsteps = 0, nlu = 0, nnewton = 0, nrej = 0, nfail = 0;
+  int nsteps = 0, nlu = 0, nnewton = 0, nrej = 0, nfail = 0, jac_step = 0;
   /* initial step (Hairer, Norsett, Wanner I, II.4) */
   sys_rhs(s, t, y0, w->f);
   double h_abs;
   if (p->first_step > 0) {
     h_abs = p->first_step;
   } else {
-    for (int i = 0; i < n; ++i) w->scale[i] = atol + fabs(y0[i]) * rtol;
+    for (int i = 0; i < n; ++i) w->scale[i] = ATOL(i) + fabs(y0[i]) * rtol;
     double d0 = rms(n, y0, w->scale), d1 = rms(n, w->f, w->scale);
     double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
     h0 = fmin(h0, t_bound);
@@ -1286,10 +1293,17 @@ static int adaptive_core(sys_t* s, const ppo_adapt_params* p, double* y0, double
         double sum = 0.0;
         for (int k = 0; k <= order; ++k) sum += D[k * n + i];
         w->ypred[i] = sum;
-        w->scale[i] = atol + rtol * fabs(sum);
+        w->scale[i] = ATOL(i) + rtol * fabs(sum);
         double ps = 0.0;
         for (int k = 1; k <= order; ++k) ps += D[k * n + i] * gamma[k];
         w->psi[i] = ps / alpha[order];
+      }
+      /* Jacobian older than jac_age steps: re-evaluated at the prediction */
+      if (p->jac_age > 0 && !current_jac && nsteps - jac_step >= p->jac_age) {
+        sys_jac(s, w->ypred, w->J);
+        lu_valid = 0;
+        current_jac = 1;
+        jac_step = nsteps;
       }
       int converged = 0;
       const double c = h / alpha[order];
@@ -1306,7 +1320,7 @@ static int adaptive_core(sys_t* s, const ppo_adapt_params* p, double* y0, double
         double dy_norm_old = -1.0;
         converged = 0;
         int k;
-        for (k = 0; k < 4; ++k) {
+        for (k = 0; k < nmax; ++k) {
           sys_rhs(s, t_new, w->y, w->f);
           nnewton++;
           int finite = 1;
@@ -1316,7 +1330,7 @@ static int adaptive_core(sys_t* s, const ppo_adapt_params* p, double* y0, double
           lu_solve(n, w->LU, w->piv, w->dy);
           double dy_norm = rms(n, w->dy, w->scale);
           double rate = dy_norm_old < 0 ? -1.0 : dy_norm / dy_norm_old;
-          if (rate >= 0 && (rate >= 1 || pow(rate, 4 - k) / (1 - rate) * dy_norm > newton_tol)) break;
+          if (rate >= 0 && (rate >= 1 || pow(rate, nmax - k) / (1 - rate) * dy_norm > newton_tol)) break;
           for (int i = 0; i < n; ++i) { w->y[i] += w->dy[i]; w->d[i] += w->dy[i]; }
           if (dy_norm == 0 || (rate >= 0 && rate / (1 - rate) * dy_norm < newton_tol)) {
             converged = 1;
@@ -1330,6 +1344,7 @@ static int adaptive_core(sys_t* s, const ppo_adapt_params* p, double* y0, double
           sys_jac(s, w->ypred, w->J);
           lu_valid = 0;
           current_jac = 1;
+          jac_step = nsteps;
         }
       }
       if (!converged) {
@@ -1340,9 +1355,9 @@ static int adaptive_core(sys_t* s, const ppo_adapt_params* p, double* y0, double
         nfail++;
         continue;
       }
-      safety = 0.9 * (2 * 4 + 1) / (double)(2 * 4 + n_iter);
+      safety = 0.9 * (2 * nmax + 1) / (double)(2 * nmax + n_iter);
       for (int i = 0; i < n; ++i) {
-        w->scale[i] = atol + rtol * fabs(w->y[i]);
+        w->scale[i] = ATOL(i) + rtol * fabs(w->y[i]);
         w->tmp[i] = econst[order] * w->d[i];
       }
       error_norm = rms(n, w->tmp, w->scale);
@@ -1371,7 +1386,7 @@ static int adaptive_core(sys_t* s, const ppo_adapt_params* p, double* y0, double
       for (int i = 0; i < n; ++i) w->tmp[i] = econst[order - 1] * D[order * n + i];
       em = rms(n, w->tmp, w->scale);
     }
-    if (order < AMAXORD) {
+    if (order < maxord) {
       for (int i = 0; i < n; ++i) w->tmp[i] = econst[order + 1] * D[(order + 2) * n + i];
       ep = rms(n, w->tmp, w->scale);
     }
@@ -1390,6 +1405,7 @@ static int adaptive_core(sys_t* s, const ppo_adapt_params* p, double* y0, double
     lu_valid = 0;
   }
 done:
+#undef ATOL
   for (int i = 0; i < n; ++i) y0[i] = D[i];
   if (st) {
     st->nsteps += nsteps; st->nrhs += s->nrhs; st->njac += s->njac; st->nlu += nlu;

@@ -108,6 +108,10 @@ typedef struct {
   double rtol, atol;
   double first_step; /* <= 0: automatic (Hairer) */
   int max_steps;
+  int max_order;       /* 1..5; 0: 5 */
+  int newton_maxiter;  /* Newton iterations per attempt; 0: 4 */
+  int jac_age;         /* re-evaluate the Jacobian after this many steps; 0: never */
+  const double* atolv; /* per-component absolute tolerances (n), or NULL: atol */
 } ppo_adapt_params;
 
 typedef struct {

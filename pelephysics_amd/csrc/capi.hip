@@ -4,6 +4,7 @@
 // reference's int(average steps) return value.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
@@ -194,6 +195,8 @@ struct pp_chem {
   // this device's address of the file-scope scratch array of
   // pp_chem_react_1d_noalloc (null when it could not be resolved)
   unsigned char* scratch = nullptr;
+  // ReactorCvode: absolute tolerances atol x typical value per component
+  double* d_atolv = nullptr;
 };
 
 namespace {
@@ -273,10 +276,14 @@ int finish_with(pp_chem* h, int ncells, hipStream_t s, float kernel_ms, const pp
     return fail(h, PP_CHEM_EINTEGRATE,
                 std::to_string(S.nfailed) + " cell(s) failed to integrate (NaN, step limit or step too small)");
   if (ncells == 0) return 0;
-  return (int)((double)S.nsteps / (double)ncells);
+  // ReactorCvode reports RHS evaluations (CVodeGetNumRhsEvals,
+  // ReactorCvode.cpp:1365-1372), the other integrators steps
+  return (int)((double)(h->cfg.mode == PP_CHEM_CVODE ? S.nrhs : S.nsteps) / (double)ncells);
 }
 
 bool is_batched(int mode) { return mode == ppc::MODE_BATCHED_MIXED || mode == ppc::MODE_BATCHED_FP64; }
+// Kernels of a mode: ReactorCvode runs the plain FP64 adaptive BDF with its options.
+int kmode(int mode) { return mode == PP_CHEM_CVODE ? ppc::MODE_ADAPTIVE : mode; }
 
 // ---- device scratch of pp_chem_react_1d_noalloc: one array per GPU, held
 // until the process ends.  Layout: [0, 256) queue counter, [256, 512) the
@@ -316,7 +323,7 @@ int prepare_workspace(pp_chem* h, int ncells) {
   h->P.ws = nullptr;
   h->P.ws_lanes = 0;
   const bool group = is_batched(h->cfg.mode) && h->cfg.batched_cells_per_wave != 0;
-  const long long rows = group ? h->L->group_ws_rows(h->cfg.batched_cells_per_wave) : h->L->ws_rows(h->cfg.mode);
+  const long long rows = group ? h->L->group_ws_rows(h->cfg.batched_cells_per_wave) : h->L->ws_rows(kmode(h->cfg.mode));
   if (rows == 0 || ncells == 0) return 0;
   int ncu = 0;
   HIPCHK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
@@ -375,7 +382,7 @@ hipError_t enqueue_1d(pp_chem* h, double* y, double* ysrc, double* rX, double* r
     const ppc::QueueArgs q = prepare_queue(h, ncells, key, s);
     e = h->L->batched_1d(h->cfg.mode, h->P, y, ysrc, rX, rXsrc, dt, q, counters(h), s);
   } else {
-    e = h->L->react_1d(h->cfg.mode, h->P, y, ysrc, rX, rXsrc, dt, ncells, counters(h), h->cfg.block_size, s);
+    e = h->L->react_1d(kmode(h->cfg.mode), h->P, y, ysrc, rX, rXsrc, dt, ncells, counters(h), h->cfg.block_size, s);
   }
   if (e != hipSuccess || !count_fail || ncells == 0) return e;
   const int blocks = std::min(1024, (ncells + 255) / 256);
@@ -397,7 +404,8 @@ ppc::ReactParamsDev to_dev(const pp_chem_config& c) {
            c.bdf_gmres_precond, c.bdf_scheme, c.bdf_nonlinear_tol, c.bdf_gmres_tol};
   P.rk64 = {c.rk64_atol, c.rk64_nsubsteps_guess, c.rk64_nsubsteps_min, c.rk64_nsubsteps_max};
   P.adapt = {c.rtol, c.atol, c.first_step, c.max_steps, c.batched_lu_batch, c.batched_lu_wait,
-             c.batched_waves_per_simd, c.batched_cells_per_wave};
+             c.batched_waves_per_simd, c.batched_cells_per_wave, c.cvode_max_order, c.max_nls_iters,
+             c.jac_max_age, c.mode == PP_CHEM_CVODE ? 1 : 0, nullptr};
   return P;
 }
 
@@ -492,6 +500,17 @@ int pp_chem_config_init(pp_chem_config* cfg, const char* mechanism, int mode, in
   cfg->first_step = 0.0;
   cfg->max_steps = 100000;
   cfg->block_size = 0;
+  if (mode == PP_CHEM_CVODE) {
+    // ReactorCvode defaults (ReactorCvode.H:179-192, ReactorCvode.cpp:214-218):
+    // rtol 1e-6, atol 1e-10, cvode.max_order 2, cvode.max_substeps 10000,
+    // ode.max_nls_iters 3, Jacobian re-evaluated after 100 steps
+    cfg->rtol = 1e-6;
+    cfg->atol = 1e-10;
+    cfg->max_steps = 10000;
+    cfg->cvode_max_order = 2;
+    cfg->max_nls_iters = 3;
+    cfg->jac_max_age = 100;
+  }
   return PP_CHEM_OK;
 }
 
@@ -502,7 +521,13 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
   if (!L) return PP_CHEM_EMECH;
   // ReactorTypes::check_reactor_type (ReactorTypes.H:12-19) as an error code
   if (cfg->reactor_type != 1 && cfg->reactor_type != 2) return PP_CHEM_EINVAL;
-  if (cfg->mode < PP_CHEM_BDF || cfg->mode > PP_CHEM_BDF_BATCHED_FP64) return PP_CHEM_EINVAL;
+  if (cfg->mode < PP_CHEM_BDF || cfg->mode > PP_CHEM_CVODE) return PP_CHEM_EINVAL;
+  // CVODE-style options: the plain FP64 adaptive BDF and ReactorCvode only
+  if (cfg->cvode_max_order < 0 || cfg->cvode_max_order > 5 || cfg->max_nls_iters < 0 || cfg->jac_max_age < 0)
+    return PP_CHEM_EINVAL;
+  if ((cfg->cvode_max_order || cfg->max_nls_iters || cfg->jac_max_age) && cfg->mode != PP_CHEM_BDF_ADAPTIVE &&
+      cfg->mode != PP_CHEM_CVODE)
+    return PP_CHEM_EINVAL;
   if (is_batched(cfg->mode) && !L->batched_ok) return PP_CHEM_EINVAL;  // form not compiled for L
   if (cfg->mode == PP_CHEM_BDF &&
       (cfg->bdf_nsubsteps < 1 || cfg->bdf_scheme < 0 || cfg->bdf_scheme > 3 ||
@@ -526,8 +551,8 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
   // cells fit in registers (no workspace); the parity integrators 0
   const int occ = cfg->batched_waves_per_simd;
   if (is_batched(cfg->mode) ? (occ < 0 || occ > 2)
-                            : (cfg->mode == PP_CHEM_BDF_ADAPTIVE
-                                   ? (occ != 0 && ((occ != 2 && occ != 4) || L->ws_rows(cfg->mode) > 0))
+                            : ((cfg->mode == PP_CHEM_BDF_ADAPTIVE || cfg->mode == PP_CHEM_CVODE)
+                                   ? (occ != 0 && ((occ != 2 && occ != 4) || L->ws_rows(kmode(cfg->mode)) > 0))
                                    : occ != 0))
     return PP_CHEM_EINVAL;
   if (cfg->batched_waves_per_simd == 1 && cfg->batched_box_direct == 1) return PP_CHEM_EINVAL;  // no such build
@@ -581,9 +606,35 @@ void pp_chem_destroy(pp_chem* h) {
   if (h->d_fail) (void)hipFree(h->d_fail);
   if (h->d_order) (void)hipFree(h->d_order);
   if (h->d_stage) (void)hipFree(h->d_stage);
+  if (h->d_atolv) (void)hipFree(h->d_atolv);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   delete h;
+}
+
+int pp_chem_set_typ_vals(pp_chem* h, const double* typ_vals, int n) {
+  if (!h) return PP_CHEM_EINVAL;
+  if (!typ_vals || n != h->L->ns + 1) return fail(h, PP_CHEM_EINVAL, "typ_vals needs NUM_SPECIES + 1 values");
+  // read by ReactorCvode only, as the reference's ReactorBDF / ReactorRK64
+  // ignore them; typ_vals[0] <= 0 restores the scalar atol
+  if (h->cfg.mode != PP_CHEM_CVODE) return PP_CHEM_OK;
+  if (!(typ_vals[0] > 0.0)) {
+    h->P.adapt.atolv = nullptr;
+    drop_graph(h);
+    return PP_CHEM_OK;
+  }
+  std::vector<double> a((size_t)n);
+  for (int k = 0; k < n; ++k) {
+    if (!(typ_vals[k] > 0.0) || !std::isfinite(typ_vals[k]))
+      return fail(h, PP_CHEM_EINVAL, "typical values must be positive and finite");
+    a[(size_t)k] = typ_vals[k] * h->cfg.atol;
+  }
+  HIPCHK(h, hipSetDevice(h->device));
+  if (!h->d_atolv) HIPCHK(h, hipMalloc(&h->d_atolv, sizeof(double) * (size_t)n));
+  HIPCHK(h, hipMemcpy(h->d_atolv, a.data(), sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+  h->P.adapt.atolv = h->d_atolv;
+  drop_graph(h);
+  return PP_CHEM_OK;
 }
 
 int pp_chem_num_species(const pp_chem* h) { return h ? h->L->ns : PP_CHEM_EINVAL; }
@@ -592,7 +643,7 @@ long long pp_chem_workspace_rows(const pp_chem* h) {
   if (!h) return PP_CHEM_EINVAL;
   if (is_batched(h->cfg.mode) && h->cfg.batched_cells_per_wave != 0)
     return h->L->group_ws_rows(h->cfg.batched_cells_per_wave);
-  return h->L->ws_rows(h->cfg.mode);
+  return h->L->ws_rows(kmode(h->cfg.mode));
 }
 
 int pp_chem_adjoint_1d(pp_chem* h, double* y, const double* rX, int ncells, int nmacro, double dt,
@@ -688,7 +739,7 @@ int pp_chem_react_box(pp_chem* h, const int lo[3], const int hi[3], const pp_arr
       const ppc::QueueArgs q = prepare_queue(h, ncells, rY->p, s);
       e = h->L->batched_box(h->cfg.mode, h->P, b, *dt_react, q, counters(h), s);
     } else {
-      e = h->L->react_box(h->cfg.mode, h->P, b, *dt_react, ncells, counters(h), h->cfg.block_size, s);
+      e = h->L->react_box(kmode(h->cfg.mode), h->P, b, *dt_react, ncells, counters(h), h->cfg.block_size, s);
     }
     if (e != hipSuccess) return hip_fail(h, e, "react_box launch");
   }
@@ -769,7 +820,7 @@ int pp_chem_react_1d_noalloc(pp_chem* h, double* y, double* ysrc, double* rX, do
   P.ws = nullptr;
   P.ws_lanes = 0;
   const bool group = is_batched(h->cfg.mode) && h->cfg.batched_cells_per_wave != 0;
-  const long long rows = group ? h->L->group_ws_rows(h->cfg.batched_cells_per_wave) : h->L->ws_rows(h->cfg.mode);
+  const long long rows = group ? h->L->group_ws_rows(h->cfg.batched_cells_per_wave) : h->L->ws_rows(kmode(h->cfg.mode));
   if (rows > 0 && ncells > 0) {
     int ncu = 0;
     HIPCHK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
@@ -794,7 +845,7 @@ int pp_chem_react_1d_noalloc(pp_chem* h, double* y, double* ysrc, double* rX, do
       const ppc::QueueArgs q{nullptr, counter, ncells, h->max_blocks};  // cells in index order
       e = h->L->batched_1d(h->cfg.mode, P, y, ysrc, rX, rXsrc, *dt_react, q, st, s);
     } else {
-      e = h->L->react_1d(h->cfg.mode, P, y, ysrc, rX, rXsrc, *dt_react, ncells, st, h->cfg.block_size, s);
+      e = h->L->react_1d(kmode(h->cfg.mode), P, y, ysrc, rX, rXsrc, *dt_react, ncells, st, h->cfg.block_size, s);
     }
     if (e != hipSuccess) return hip_fail(h, e, "react_1d_noalloc launch");
   }

@@ -9,13 +9,6 @@
 
 namespace ppc {
 
-// The one-wave-per-cell batched form of table-driven mechanisms (wave_bdf.h)
-// is compiled only with PPC_WAVE_BATCHED=1; otherwise every mechanism runs
-// the batched forms one lane per cell (batched.h).
-#ifndef PPC_WAVE_BATCHED
-#define PPC_WAVE_BATCHED 0
-#endif
-
 struct MechLauncher {
   const char* name;
   int ns, nr;

@@ -11,8 +11,8 @@
 
 namespace ppc {
 
-// The batched forms run one lane per cell (batched.h); with PPC_WAVE_BATCHED=1
-// table-driven mechanisms run them one wave per cell (wave_bdf.h) instead.
+// Table-driven mechanism tables of a mechanism type (void for the generated
+// straight-line mechanisms): the group and wave-per-cell kernels read them.
 template <class M>
 struct wave_tables {
   using type = void;
@@ -146,16 +146,6 @@ hipError_t batched_1d_mode(const ReactParamsDev& P, double* y, const double* ysr
                            double dt, QueueArgs q, CellStats st, hipStream_t s) {
   using LU_T = typename std::conditional<MODE == MODE_BATCHED_MIXED, float, double>::type;
   if (P.adapt.group > 0) return batched_group<M, LU_T>(P, y, ysrc, rX, rXsrc, dt, q, st, s);
-#if PPC_WAVE_BATCHED
-  using Tab = typename wave_tables<M>::type;
-  if constexpr (!std::is_void<Tab>::value) {
-    auto k = wave_bdf_kernel<Tab, LU_T, WaveIO1D>;
-    const int grid = resident_blocks(k, q.ncells * 64);
-    if (grid < 0) return hipErrorInvalidDevice;
-    WaveIO1D io{y, ysrc, rX, rXsrc};
-    hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
-  } else
-#endif
   {
     auto k = P.adapt.lane_occ == 1 ? batched_kernel<M, LU_T, IO1D, 1> : batched_kernel<M, LU_T, IO1D, 2>;
     const int grid = batched_grid<M, MODE>(k, P, q);
@@ -170,16 +160,6 @@ template <class M, int MODE>
 hipError_t batched_box_mode(const ReactParamsDev& P, const BoxArgs& b, double dt, QueueArgs q, CellStats st,
                             hipStream_t s) {
   using LU_T = typename std::conditional<MODE == MODE_BATCHED_MIXED, float, double>::type;
-#if PPC_WAVE_BATCHED
-  using Tab = typename wave_tables<M>::type;
-  if constexpr (!std::is_void<Tab>::value) {
-    auto k = wave_bdf_kernel<Tab, LU_T, WaveIOBox>;
-    const int grid = resident_blocks(k, q.ncells * 64);
-    if (grid < 0) return hipErrorInvalidDevice;
-    WaveIOBox io{b};
-    hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, io, dt, q, st);
-  } else
-#endif
   {
     // built at 2 waves per SIMD only: the 1-wave build of the table-driven
     // drm19 box kernel returned failed cells, so 1-wave runs take the flat route

@@ -19,6 +19,12 @@ struct AdaptParamsDev {  // adaptive BDF tolerances
   int lu_batch, lu_wait;  // batched forms: wave-shared Jacobian/LU service (0: defaults 64, 32)
   int lane_occ;           // batched forms: kernel build held to 1 or 2 waves per SIMD (0: 2)
   int group;              // batched forms: cells per wave of the group kernel (8, 4, 2, 1); 0: one cell per lane
+  // CVODE-style options of the plain FP64 form (0 / null: SciPy's behaviour)
+  int max_order;          // order cap 1..5 (cvode.max_order)
+  int newton_maxiter;     // Newton iterations per attempt (ode.max_nls_iters)
+  int jac_age;            // re-evaluate the Jacobian after this many steps
+  int fc_rhs;             // box FC and counters report RHS evaluations (ReactorCvode), not steps
+  const double* atolv;    // per-component absolute tolerances atol x typical value (device, NEQ)
 };
 
 struct ReactParamsDev {

@@ -440,6 +440,13 @@ PPC_DI int adaptive_cell(const AdaptParamsDev& p, const int reactor_type, double
   constexpr bool WS = uses_ws<M>();
   const double rtol = p.rtol, atol = p.atol;
   const double newton_tol = fmax(10 * 2.220446049250313e-16 / rtol, fmin(0.03, sqrt(rtol)));
+  // CVODE-style options (ReactorCvode.cpp:206-224, ReactorUtils.H:119-131);
+  // 0 / null keep SciPy's order 5, 4 Newton iterations, no Jacobian age, scalar atol
+  const int maxord = (p.max_order > 0 && p.max_order < AMAXORD) ? p.max_order : AMAXORD;
+  const int nmax = p.newton_maxiter > 0 ? p.newton_maxiter : 4;
+  const double* __restrict__ atolv = p.atolv;
+  auto atol_of = [&](int i) { return atolv ? atolv[i] : atol; };
+  int jac_step = 0;
   double D[(AMAXORD + 3) * n];
   double J_p[WS ? 1 : n * n];
   LU_T LU_p[WS ? 1 : n * n];
@@ -456,7 +463,7 @@ PPC_DI int adaptive_cell(const AdaptParamsDev& p, const int reactor_type, double
     h_abs = p.first_step;
   } else {
     // initial step (Hairer, Norsett & Wanner I, II.4)
-    for (int i = 0; i < n; ++i) scale[i] = atol + fabs(y0[i]) * rtol;
+    for (int i = 0; i < n; ++i) scale[i] = atol_of(i) + fabs(y0[i]) * rtol;
     const double d0 = rms_norm<n>(y0, scale), d1 = rms_norm<n>(f, scale);
     double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
     h0 = fmin(h0, t_bound);
@@ -517,8 +524,16 @@ PPC_DI int adaptive_cell(const AdaptParamsDev& p, const int reactor_type, double
         for (int k = 0; k <= order; ++k) sum += D[k * n + i];
         for (int k = 1; k <= order; ++k) ps += D[k * n + i] * bdf_gamma(k);
         ypred[i] = sum;
-        scale[i] = atol + rtol * fabs(sum);
+        scale[i] = atol_of(i) + rtol * fabs(sum);
         psi[i] = ps * inv_alpha;
+      }
+      // a Jacobian older than jac_age steps is re-evaluated at the prediction
+      if (p.jac_age > 0 && !current_jac && nsteps - jac_step >= p.jac_age) {
+        bdf_jacobian<M>(reactor_type, ypred, 1.0, J, J);
+        njac++;
+        lu_valid = false;
+        current_jac = true;
+        jac_step = nsteps;
       }
       bool converged = false;
       const double c = h * inv_alpha;
@@ -544,7 +559,7 @@ PPC_DI int adaptive_cell(const AdaptParamsDev& p, const int reactor_type, double
         }
         double dy_norm_old = -1.0;
         int k;
-        for (k = 0; k < 4; ++k) {
+        for (k = 0; k < nmax; ++k) {
           f_kernel_spec<M>(t_new, reactor_type, yv, f, rhoe_init, rhoesrc, rYsrc);
           nrhs++;
           nnewton++;
@@ -555,7 +570,7 @@ PPC_DI int adaptive_cell(const AdaptParamsDev& p, const int reactor_type, double
           lu_solve<n, LU_T>(LU, piv, dy);
           const double dy_norm = rms_norm<n>(dy, scale);
           const double rate = dy_norm_old < 0 ? -1.0 : dy_norm / dy_norm_old;
-          if (rate >= 0 && (rate >= 1 || pow(rate, 4 - k) / (1 - rate) * dy_norm > newton_tol)) break;
+          if (rate >= 0 && (rate >= 1 || pow(rate, nmax - k) / (1 - rate) * dy_norm > newton_tol)) break;
           for (int i = 0; i < n; ++i) {
             yv[i] += dy[i];
             d[i] += dy[i];
@@ -573,6 +588,7 @@ PPC_DI int adaptive_cell(const AdaptParamsDev& p, const int reactor_type, double
           njac++;
           lu_valid = false;
           current_jac = true;
+          jac_step = nsteps;
         }
       }
       if (!converged) {
@@ -583,9 +599,9 @@ PPC_DI int adaptive_cell(const AdaptParamsDev& p, const int reactor_type, double
         nrej++;
         continue;
       }
-      safety = 0.9 * 9.0 / (double)(8 + n_iter);
+      safety = 0.9 * (2 * nmax + 1) / (double)(2 * nmax + n_iter);
       for (int i = 0; i < n; ++i) {
-        scale[i] = atol + rtol * fabs(yv[i]);
+        scale[i] = atol_of(i) + rtol * fabs(yv[i]);
         dy[i] = ndf_econst(order) * d[i];
       }
       error_norm = rms_norm<n>(dy, scale);
@@ -615,7 +631,7 @@ PPC_DI int adaptive_cell(const AdaptParamsDev& p, const int reactor_type, double
       for (int i = 0; i < n; ++i) dy[i] = ndf_econst(order - 1) * D[order * n + i];
       em = rms_norm<n>(dy, scale);
     }
-    if (order < AMAXORD) {
+    if (order < maxord) {
       for (int i = 0; i < n; ++i) dy[i] = ndf_econst(order + 1) * D[(order + 2) * n + i];
       ep = rms_norm<n>(dy, scale);
     }
@@ -716,7 +732,7 @@ PPC_DI void react_box_cells(const ReactParamsDev& P, const BoxArgs& b, double dt
     const double rE_new = rhoe + dt * rhoesrc;
     b.rE(i, j, k, 0) = rE_new;
     b.T(i, j, k, 0) = T_from_energy_rt<M>(P.reactor_type, rE_new / rho, Y, y[NS]);
-    if (b.FC.p) b.FC(i, j, k, 0) = (double)r;
+    if (b.FC.p) b.FC(i, j, k, 0) = (double)(P.adapt.fc_rhs && r >= 0 ? cnt[0] : r);
     store_counters(st, c, r, cnt);
   }
 }

@@ -1,11 +1,6 @@
-// Batched adaptive BDF with one 64-lane wavefront per cell.
-//
-// Lane i < NEQ holds equation i of the cell (rhoY_i, then T): its entries of
-// the BDF difference array, the iterate, the error weights, row i of the
-// Jacobian and row i of the LU factors of the Newton matrix, all in
-// registers.  The integrator's control flow (step size, order, Newton
-// convergence, error test) depends only on wave-wide reductions, so it is
-// uniform: a wave never executes a branch that its cell does not take.
+// Wave-per-cell chemistry for the discrete adjoint (wave_adjoint.h): one
+// 64-lane wavefront per cell, lane i < NEQ holding equation i (rhoY_i, then
+// T) with its row of the Jacobian and of the LU factors in registers.
 //
 // Right-hand side: each species lane evaluates its NASA polynomials and
 // exp(g/RT); concentrations and exponentials go to LDS; reactions are spread
@@ -18,14 +13,13 @@
 // (ds_add_f64) into an NEQ x NEQ matrix, then the temperature row, then each
 // lane reads its row with the molecular-weight scaling of ReactorBDF.cpp:67-82.
 //
-// Newton matrix: S^-1 (I - c J) S with S the error weights, factored in LU_T
-// (float: FP32 LU, FP64 residuals and rates) by row-per-lane Gaussian
-// elimination with partial pivoting: the pivot is a wave max-reduction, the
-// pivot row reaches the other lanes through v_readlane, and rows never move
-// (each lane remembers the elimination step at which it was the pivot).
+// LU: row-per-lane Gaussian elimination with partial pivoting: the pivot is
+// a wave max-reduction, the pivot row reaches the other lanes through
+// v_readlane, and rows never move (each lane remembers the elimination step
+// at which it was the pivot).
 //
-// The step/order/Newton logic is the NDF family of adaptive_cell (reactors.h),
-// itself the algorithm of SciPy's BDF integrator.
+// (The batched adaptive integrators run several cells per wave, or one, in
+// group_bdf.h.)
 #pragma once
 #include "chem_table.h"
 #include "reactors.h"
@@ -344,405 +338,6 @@ PPC_DI double wave_lu_solve(const LT* a, int lane, int kstep, int pivlane, LT di
     if (kstep < k) b -= (double)a[k] * xk;
   }
   return x;
-}
-
-PPC_DI double wave_rms(double x, double sc, bool act, int n) {
-  const double v = act ? x / sc : 0.0;
-  return sqrt(wave::sum(v * v) / n);
-}
-
-// D[0..order] <- (R U)^T D[0..order] for a step-size ratio `factor` (change_D
-// in reactors.h).  Lane 6i+j forms RU[i][j] = sum_k R[i][k] U[k][j] with
-// R[i][k] = prod_{m=1..i} (m-1-factor k)/m and U = R at factor 1, in the
-// order of the sequential form; the entries reach every lane by v_readlane.
-PPC_DI void wave_change_D(double* D, const int order, const double factor, const int lane) {
-  const int i = lane / (AMAXORD + 1), j = lane % (AMAXORD + 1);
-  double ru = 0.0;
-  if (i <= order && j <= order) {
-#pragma unroll 1
-    for (int k = 0; k <= order; ++k) {
-      double r = 1.0, u = 1.0;
-#pragma unroll 1
-      for (int m = 1; m <= i; ++m) r *= (m - 1 - factor * k) / (double)m;
-#pragma unroll 1
-      for (int m = 1; m <= k; ++m) u *= (m - 1 - 1.0 * j) / (double)m;
-      ru += r * u;
-    }
-  }
-  double tmp[AMAXORD + 1];
-#pragma unroll
-  for (int c = 0; c <= AMAXORD; ++c) {
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k <= AMAXORD; ++k)
-      if (k <= order) s += wave::rdlane(ru, k * (AMAXORD + 1) + c) * D[k];
-    tmp[c] = s;
-  }
-#pragma unroll
-  for (int c = 0; c <= AMAXORD; ++c)
-    if (c <= order) D[c] = tmp[c];
-}
-
-enum : int { WPH_F0 = 0, WPH_F1 = 1, WPH_NEWTON = 2 };
-
-// Adaptive BDF for the cell of this wave from t = 0 to t_bound; y in/out
-// (this lane's equation).  Returns the accepted step count or a negative
-// status.  Written as a loop with one right-hand side per trip (initial-step
-// probes, then Newton iterations) so that the RHS, the Jacobian, the LU and
-// change_D each have a single call site in the kernel.
-template <class T, typename LT>
-PPC_DI int wave_cell(const AdaptParamsDev& ap, const int reactor_type, const typename WaveChem<T>::Lane& L,
-                     typename WaveChem<T>::template Shared<LT>& sh, const int lane, double& y, const double src,
-                     const double rhoe, const double rhoesrc, const double t_bound, int* cnt) {
-  using W = WaveChem<T>;
-  constexpr int n = W::NEQ;
-  const bool act = lane < n;
-  const double rtol = ap.rtol, atol = ap.atol;
-  const double newton_tol = fmax(10 * 2.220446049250313e-16 / rtol, fmin(0.03, sqrt(rtol)));
-  double D[AMAXORD + 3];
-#pragma unroll
-  for (int k = 0; k < AMAXORD + 3; ++k) D[k] = 0.0;
-  LT LU[n];
-#pragma unroll
-  for (int k = 0; k < n; ++k) LU[k] = LT(0);
-  int kstep = 0, pivlane = 0;
-  LT dinv = LT(0);
-  const double y0 = act ? y : 0.0;
-  D[0] = y0;
-  double ypred = 0, scale = 1, psi = 0, yv = y0, d = 0, f0 = 0, lus = 1;
-  int nsteps = 0, nrhs = 0, njac = 0, nlu = 0, nnewton = 0, nrej = 0, status = 0;
-  double t = 0.0, t_new = 0.0, t_eval = 0.0, h_abs = 0.0, h0 = 0.0, c = 0.0, dy_norm_old = -1.0;
-  double min_step = 0.0, safety = 0.9;
-  int order = 1, n_equal = 0, k_newton = 0, phase = WPH_F0;
-  bool lu_valid = false, current_jac = false;
-  // every path through the loop either accepts a step (bounded by max_steps)
-  // or shrinks h until the minimum-step test fails; the RHS budget is a
-  // second bound against non-finite step sizes
-  const long long rhs_cap = 64LL * ap.max_steps + 4096;
-  for (;;) {
-    if (nrhs >= rhs_cap) {
-      status = -2;
-      break;
-    }
-    const double f = W::rhs(L, sh, lane, t_eval, reactor_type, yv, src, rhoe, rhoesrc);
-    nrhs++;
-    // what to prepare before the next right-hand side
-    bool jac = false, jac_pred = false, new_step = false, new_attempt = false, retry = false, cd = false;
-    double cd_fac = 1.0;
-    if (phase == WPH_F0) {
-      f0 = f;
-      if (ap.first_step > 0) {
-        h_abs = ap.first_step;
-        jac = new_step = true;
-      } else {
-        // initial step (Hairer, Norsett & Wanner I, II.4)
-        scale = atol + fabs(y0) * rtol;
-        const double d0 = wave_rms(y0, scale, act, n), d1 = wave_rms(f, scale, act, n);
-        h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
-        h0 = fmin(h0, t_bound);
-        yv = y0 + h0 * f;
-        t_eval = t + h0;
-        phase = WPH_F1;
-        continue;
-      }
-    } else if (phase == WPH_F1) {
-      const double d1 = wave_rms(f0, scale, act, n);
-      const double d2 = wave_rms(f - f0, scale, act, n) / h0;
-      const double h1 = (d1 <= 1e-15 && d2 <= 1e-15) ? fmax(1e-6, h0 * 1e-3) : sqrt(0.01 / fmax(d1, d2));
-      h_abs = fmin(fmin(100 * h0, h1), t_bound);
-      jac = new_step = true;
-    } else {
-      // one modified-Newton iteration with f = f(t_new, yv)
-      nnewton++;
-      const bool finite = wave::ballot(act && !isfinite(f)) == 0ull;
-      bool failed = !finite, converged = false;
-      if (finite) {
-        double dy = (c * f - psi - d) / lus;
-        dy = wave_lu_solve<n, LT>(LU, lane, kstep, pivlane, dinv, act ? dy : 0.0);
-        dy *= lus;
-        const double dy_norm = wave_rms(dy, scale, act, n);
-        const double rate = dy_norm_old < 0 ? -1.0 : dy_norm / dy_norm_old;
-        if (rate >= 0 && (rate >= 1 || pow(rate, 4 - k_newton) / (1 - rate) * dy_norm > newton_tol)) {
-          failed = true;
-        } else {
-          yv += dy;
-          d += dy;
-          if (dy_norm == 0 || (rate >= 0 && rate / (1 - rate) * dy_norm < newton_tol)) {
-            converged = true;
-          } else {
-            dy_norm_old = dy_norm;
-            k_newton++;
-            if (k_newton == 4) failed = true;
-          }
-        }
-      }
-      const int n_iter = k_newton + 1;
-      if (failed) {
-        if (!current_jac) {
-          // refresh the Jacobian at the predictor and retry the same step
-          jac = jac_pred = retry = true;
-          current_jac = true;
-          lu_valid = false;
-        } else {
-          h_abs *= 0.5;
-          cd = true;
-          cd_fac = 0.5;
-          n_equal = 0;
-          lu_valid = false;
-          nrej++;
-          new_attempt = true;
-        }
-      } else if (converged) {
-        safety = 0.9 * 9.0 / (double)(8 + n_iter);
-        scale = atol + rtol * fabs(yv);
-        const double error_norm = wave_rms(ndf_econst(order) * d, scale, act, n);
-        if (error_norm > 1) {
-          const double fac = fmax(0.2, safety * pow(error_norm, -1.0 / (order + 1)));
-          h_abs *= fac;
-          cd = true;
-          cd_fac = fac;
-          n_equal = 0;
-          nrej++;
-          new_attempt = true;
-        } else {
-          // accept: D[order+2] = d - D[order+1]; D[order+1] = d; D[k] += D[k+1], k = order..0
-          nsteps++;
-          n_equal++;
-          t = t_new;
-          double Dn[AMAXORD + 3];
-#pragma unroll
-          for (int k = 0; k < AMAXORD + 3; ++k) Dn[k] = D[k];
-#pragma unroll
-          for (int k = 1; k < AMAXORD + 3; ++k) {
-            if (k == order + 2) Dn[k] = d - D[k - 1];
-            if (k == order + 1) Dn[k] = d;
-          }
-#pragma unroll
-          for (int k = AMAXORD + 1; k >= 0; --k)
-            if (k <= order) Dn[k] += Dn[k + 1];
-#pragma unroll
-          for (int k = 0; k < AMAXORD + 3; ++k) D[k] = Dn[k];
-          if (n_equal >= order + 1) {
-            double em = INFINITY, ep = INFINITY, Dm = 0.0, Dp = 0.0;
-#pragma unroll
-            for (int k = 0; k < AMAXORD + 3; ++k) {
-              if (k == order) Dm = D[k];
-              if (k == order + 2) Dp = D[k];
-            }
-            if (order > 1) em = wave_rms(ndf_econst(order - 1) * Dm, scale, act, n);
-            if (order < AMAXORD) ep = wave_rms(ndf_econst(order + 1) * Dp, scale, act, n);
-            const double norms[3] = {em, error_norm, ep};
-            double best = -1.0;
-            int delta = 0;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-              const double fct = (norms[i] == 0.0) ? INFINITY : pow(norms[i], -1.0 / (order + i));
-              if (fct > best) {
-                best = fct;
-                delta = i - 1;
-              }
-            }
-            order += delta;
-            const double fac = fmin(10.0, safety * best);
-            h_abs *= fac;
-            cd = true;
-            cd_fac = fac;
-            n_equal = 0;
-            lu_valid = false;
-          }
-          if (!(t < t_bound)) break;
-          new_step = true;
-        }
-      }
-    }
-    // ---- prepare the next right-hand side
-    if (jac) {
-      W::jacobian_rows_rt(L, sh, lane, reactor_type, jac_pred ? ypred : D[0]);
-      njac++;
-    }
-    if (phase != WPH_NEWTON) {
-      // first step: D[1] = h f0
-      D[1] = f0 * h_abs;
-    }
-    // the change_D calls of one transition, in order: the step-size change of
-    // the decision, the minimum-step clamp at a new step, the clamp to t_bound
-    for (int stage = 0; stage < 3; ++stage) {
-      bool doit = false;
-      double fac = 1.0;
-      if (stage == 0) {
-        doit = cd;
-        fac = cd_fac;
-      } else if (stage == 1) {
-        if (new_step) {
-          if (nsteps >= ap.max_steps) {
-            status = -2;
-            break;
-          }
-          min_step = 10 * fabs(nextafter(t, INFINITY) - t);
-          if (h_abs < min_step) {
-            doit = true;
-            fac = min_step / h_abs;
-            h_abs = min_step;
-            n_equal = 0;
-          }
-          current_jac = false;
-        }
-      } else if (new_step || new_attempt) {
-        if (h_abs < min_step) {
-          status = -3;
-          break;
-        }
-        t_new = t + h_abs;
-        if (t_new - t_bound > 0) {
-          t_new = t_bound;
-          doit = true;
-          fac = fabs(t_new - t) / h_abs;
-          n_equal = 0;
-          lu_valid = false;
-        }
-      }
-      if (doit) wave_change_D(D, order, fac, lane);
-    }
-    if (status != 0) break;
-    if (new_step || new_attempt) {
-      const double h = t_new - t;
-      h_abs = fabs(h);
-      const double inv_alpha = 1.0 / ndf_alpha(order);
-      double sum = 0.0, ps = 0.0;
-#pragma unroll
-      for (int k = 0; k <= AMAXORD; ++k)
-        if (k <= order) sum += D[k];
-#pragma unroll
-      for (int k = 1; k <= AMAXORD; ++k)
-        if (k <= order) ps += D[k] * bdf_gamma(k);
-      ypred = sum;
-      scale = atol + rtol * fabs(sum);
-      psi = ps * inv_alpha;
-      c = h * inv_alpha;
-    }
-    if (new_step || new_attempt || retry) {
-      if (!lu_valid) {
-        // S^-1 (I - c J) S with S the current error weights
-        lus = scale;
-        const double si = 1.0 / lus;
-#pragma unroll
-        for (int j = 0; j < n; ++j) {
-          const double sj = wave::rdlane(lus, j);
-          const double m = (lane == j ? 1.0 : 0.0) - c * (act ? (double)sh.Jrow[lane][j] : 0.0);
-          LU[j] = (LT)(si * m * sj);
-        }
-        wave_lu_factor<n, LT>(LU, lane, kstep, pivlane, dinv);
-        nlu++;
-        lu_valid = true;
-      }
-      yv = ypred;
-      d = 0.0;
-      dy_norm_old = -1.0;
-      k_newton = 0;
-    }
-    t_eval = t_new;
-    phase = WPH_NEWTON;
-  }
-  y = D[0];
-  cnt[0] = nrhs;
-  cnt[1] = njac;
-  cnt[2] = nlu;
-  cnt[3] = nnewton;
-  cnt[4] = nrej;
-  return status == 0 ? nsteps : status;
-}
-
-// Cell input/output of the 1-D (YCOrder) form, one wave per cell.
-struct WaveIO1D {
-  double* y;
-  const double* ysrc;
-  double* rX;
-  const double* rXsrc;
-  template <class W, class SH>
-  PPC_DI void load(int c, int lane, const typename W::Lane& L, const SH&, int, double& y0, double& src,
-                   double& rhoe, double& rhoesrc) const {
-    y0 = lane < W::NEQ ? y[(long long)c * W::NEQ + lane] : 0.0;
-    src = lane < W::NS ? ysrc[(long long)c * W::NS + lane] : 0.0;
-    rhoe = rX[c];
-    rhoesrc = rXsrc[c];
-  }
-  template <class W, class SH>
-  PPC_DI void store(int c, int lane, const typename W::Lane& L, const SH&, int, double yv, double rhoe,
-                    double rhoesrc, double dt, int) const {
-    if (lane < W::NEQ) y[(long long)c * W::NEQ + lane] = yv;
-    if (lane == 0) rX[c] = rhoe + dt * rhoesrc;
-  }
-};
-
-// Cell input/output of the box (Array4) form with the initial and final
-// temperature inversions of ReactorBDF.cpp:376-389,461-478.
-struct WaveIOBox {
-  BoxArgs b;
-  PPC_DI void ijk(int c, int& i, int& j, int& k) const {
-    i = b.lo[0] + c % b.len[0];
-    j = b.lo[1] + (c / b.len[0]) % b.len[1];
-    k = b.lo[2] + c / (b.len[0] * b.len[1]);
-  }
-  template <class W, class SH>
-  PPC_DI void load(int c, int lane, const typename W::Lane& L, const SH& sh, int reactor_type, double& y0,
-                   double& src, double& rhoe, double& rhoesrc) const {
-    int i, j, k;
-    ijk(c, i, j, k);
-    const double rY = L.sp ? b.rY(i, j, k, lane) : 0.0;
-    src = L.sp ? b.rYsrc(i, j, k, lane) : 0.0;
-    const double rho = wave::sum(rY);
-    const double Y = rY * (1.0 / rho);
-    rhoe = b.rE(i, j, k, 0);
-    rhoesrc = b.rEsrc(i, j, k, 0);
-    const double T0 = W::T_from_energy_rt(L, sh, reactor_type, rhoe / rho, Y, b.T(i, j, k, 0));
-    y0 = L.sp ? rY : (lane == W::NS ? T0 : 0.0);
-  }
-  template <class W, class SH>
-  PPC_DI void store(int c, int lane, const typename W::Lane& L, const SH& sh, int reactor_type, double yv,
-                    double rhoe, double rhoesrc, double dt, int nsteps) const {
-    int i, j, k;
-    ijk(c, i, j, k);
-    const double rY = L.sp ? yv : 0.0;
-    const double rho = wave::sum(rY);
-    const double Y = rY * (1.0 / rho);
-    if (L.sp) b.rY(i, j, k, lane) = yv;
-    const double rE_new = rhoe + dt * rhoesrc;
-    const double Tn = W::T_from_energy_rt(L, sh, reactor_type, rE_new / rho, Y, wave::rdlane(yv, W::NS));
-    if (lane == 0) {
-      b.rE(i, j, k, 0) = rE_new;
-      b.T(i, j, k, 0) = Tn;
-      if (b.FC.p) b.FC(i, j, k, 0) = (double)nsteps;
-    }
-  }
-};
-
-// Waves per SIMD the register allocation is held to.
-#ifndef PPC_WAVE_OCC
-#define PPC_WAVE_OCC __attribute__((amdgpu_waves_per_eu(2)))
-#endif
-
-// Persistent kernel: each wave takes cells from the queue until it is empty.
-template <class T, typename LT, class IO>
-__global__ void __launch_bounds__(64) PPC_WAVE_OCC wave_bdf_kernel(ReactParamsDev P, IO io, double dt, QueueArgs q,
-                                                      CellStats st) {
-  using W = WaveChem<T>;
-  __shared__ typename W::template Shared<LT> sh;
-  const int lane = wave::lane_id();
-  typename W::Lane L;
-  W::load_lane(L, sh, lane);
-  for (;;) {
-    int cell = 0;
-    if (lane == 0) cell = atomicAdd(q.counter, 1);
-    cell = wave::rdlane(cell, 0);
-    if (cell >= q.ncells) break;
-    const int cid = q.order ? q.order[cell] : cell;
-    double y, src, rhoe, rhoesrc;
-    io.template load<W>(cid, lane, L, sh, P.reactor_type, y, src, rhoe, rhoesrc);
-    int cnt[5];
-    const int r = wave_cell<T, LT>(P.adapt, P.reactor_type, L, sh, lane, y, src, rhoe, rhoesrc, dt, cnt);
-    io.template store<W>(cid, lane, L, sh, P.reactor_type, y, rhoe, rhoesrc, dt, r);
-    if (lane == 0) store_counters(st, cid, r, cnt);
-  }
 }
 
 }  // namespace ppc

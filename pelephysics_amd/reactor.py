@@ -3,7 +3,8 @@
 Reference interface (Source/Reactions/ReactorBase.H:14-106, Source/Factory.H):
 
 * ``ReactorBase.create(name)`` selects an integrator by its identifier
-  ("ReactorBDF", "ReactorRK64") plus this project's adaptive forms
+  ("ReactorBDF", "ReactorRK64", "ReactorCvode": ReactorCvode's options on
+  this project's FP64 adaptive BDF) plus this project's adaptive forms
   ("ReactorBDFAdaptive", "ReactorBDFAdaptiveMixed", "ReactorBDFBatchedFP64");
   unknown names raise with
   the list of valid names (``Factory.H:135-147`` aborts with the same list);
@@ -14,8 +15,8 @@ Reference interface (Source/Reactions/ReactorBase.H:14-106, Source/Factory.H):
   ``react`` overloads; arrays are torch tensors on the GPU (updated in place)
   or, for the 1-D form, host numpy arrays (copied in and out as the
   reference's GPU build does);
-* the return value is ``int(average steps)``; errors raise ``ReactorError``
-  (the reference aborts).
+* the return value is ``int(average steps)`` (ReactorCvode: average RHS
+  evaluations); errors raise ``ReactorError`` (the reference aborts).
 
 The computation is done by ``libpele_chem_hip.so`` (HIP kernels for gfx950);
 there is no CPU fallback: a missing library raises.
@@ -32,7 +33,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # the in-tree build; PELE_CHEM_HIP_LIB selects another build of the same library
 LIB_PATH = os.environ.get("PELE_CHEM_HIP_LIB") or os.path.join(HERE, "libpele_chem_hip.so")
 
-BDF, RK64, BDF_ADAPTIVE, BDF_ADAPTIVE_MIXED, BDF_BATCHED_FP64 = 0, 1, 2, 3, 4
+BDF, RK64, BDF_ADAPTIVE, BDF_ADAPTIVE_MIXED, BDF_BATCHED_FP64, CVODE = 0, 1, 2, 3, 4, 5
 E_REACTOR, H_REACTOR = 1, 2  # Source/Reactions/ReactorTypes.H:10-11
 
 IDENTIFIERS = {
@@ -41,6 +42,7 @@ IDENTIFIERS = {
     "ReactorBDFAdaptive": BDF_ADAPTIVE,
     "ReactorBDFAdaptiveMixed": BDF_ADAPTIVE_MIXED,
     "ReactorBDFBatchedFP64": BDF_BATCHED_FP64,
+    "ReactorCvode": CVODE,
 }
 
 ERRORS = {-1: "EINVAL", -2: "EMECH", -3: "EHIP", -4: "EINTEGRATE"}
@@ -67,6 +69,7 @@ class Config(C.Structure):
         ("clean_init_massfrac", C.c_int), ("batched_waves_per_cu", C.c_int),
         ("batched_lu_batch", C.c_int), ("batched_lu_wait", C.c_int), ("batched_box_direct", C.c_int),
         ("batched_waves_per_simd", C.c_int), ("batched_cells_per_wave", C.c_int),
+        ("cvode_max_order", C.c_int), ("max_nls_iters", C.c_int), ("jac_max_age", C.c_int),
     ]
 
 
@@ -101,6 +104,10 @@ _KEYS = {
     "batched_lu_batch": "batched_lu_batch", "batched_lu_wait": "batched_lu_wait",
     "batched_box_direct": "batched_box_direct", "batched_waves_per_simd": "batched_waves_per_simd",
     "batched_cells_per_wave": "batched_cells_per_wave",
+    # ReactorCvode (and the plain adaptive BDF): cvode.max_order,
+    # cvode.max_substeps, ode.max_nls_iters, the Jacobian's largest age
+    "max_order": "cvode_max_order", "max_substeps": "max_steps", "max_nls_iters": "max_nls_iters",
+    "jac_max_age": "jac_max_age",
 }
 
 _lib = None
@@ -126,6 +133,7 @@ def library() -> C.CDLL:
         L.pp_chem_config_init.argtypes = [C.POINTER(Config), C.c_char_p, C.c_int, C.c_int]
         L.pp_chem_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
         L.pp_chem_destroy.argtypes = [vp]
+        L.pp_chem_set_typ_vals.argtypes = [vp, C.POINTER(C.c_double), C.c_int]
         L.pp_chem_num_species.argtypes = [vp]
         L.pp_chem_num_reactions.argtypes = [vp]
         L.pp_chem_workspace_rows.argtypes = [vp]
@@ -261,13 +269,18 @@ class ReactorBase:
 
     def set_typ_vals_ode(self, typ_vals):
         """ReactorBase::set_typ_vals_ode (ReactorBase.cpp:6-31): N+1 typical
-        values (species, then T).  Like the reference's ReactorBDF and
-        ReactorRK64, no integrator here reads them (only CVODE / ARKODE
-        tolerances do); they are kept for callers that set them."""
+        values (species, then T).  ReactorCvode then takes atol x typ_vals[k]
+        as the absolute tolerance of component k (ReactorUtils.H:119-131);
+        like the reference's ReactorBDF and ReactorRK64, the other integrators
+        do not read them."""
         vals = [float(v) for v in typ_vals]
         if len(vals) != self.nspec + 1:
             raise ReactorError(-1, f"set_typ_vals_ode: expected {self.nspec + 1} values, got {len(vals)}")
         self.typ_vals = vals
+        arr = (C.c_double * len(vals))(*vals)
+        rc = library().pp_chem_set_typ_vals(self._h, arr, len(vals))
+        if rc != 0:
+            raise ReactorError(rc, library().pp_chem_last_error(self._h).decode())
 
     def set_eos_parm(self, eosparm):
         """ReactorBase::set_eos_parm (ReactorBase.H:86-97): a parameter block
@@ -536,5 +549,15 @@ class ReactorBDFBatchedFP64(ReactorBase):
     _mode = BDF_BATCHED_FP64
 
 
+class ReactorCvode(ReactorBase):
+    """ReactorCvode's options and defaults (ReactorCvode.H:179-192: rtol 1e-6,
+    atol 1e-10 x typical values, order <= 2, 3 Newton iterations, Jacobian
+    age <= 100 steps, 10000 steps) on the FP64 adaptive BDF, one cell at a
+    time as the reference's CPU path integrates; FC and the return value
+    count RHS evaluations."""
+    identifier = "ReactorCvode"
+    _mode = CVODE
+
+
 _REGISTRY = {c.identifier: c for c in (ReactorBDF, ReactorRK64, ReactorBDFAdaptive,
-                                       ReactorBDFAdaptiveMixed, ReactorBDFBatchedFP64)}
+                                       ReactorBDFAdaptiveMixed, ReactorBDFBatchedFP64, ReactorCvode)}

@@ -57,7 +57,18 @@ struct ParmParse {
   std::string pre;
 };
 [[noreturn]] inline void Abort(const std::string& msg) { throw std::runtime_error(msg); }
+template <class T, int N>
+struct GpuArray {
+  T arr[N];
+  T& operator[](int i) { return arr[i]; }
+  const T& operator[](int i) const { return arr[i]; }
+};
 }  // namespace amrex
+
+// the species count of the compiled mechanism (mechanism.H); LiDryer here
+#ifndef NUM_SPECIES
+#define NUM_SPECIES 9
+#endif
 
 #define AMREX_ALWAYS_ASSERT(x) \
   do {                         \
@@ -136,8 +147,17 @@ class ReactorBase : public Factory<ReactorBase> {
                          amrex::Real* y_vect, amrex::Real* vect_energy, long int* FCunt, amrex::Real dt) = 0;
   ~ReactorBase() override = default;
 
+ public:
+  // ReactorBase.cpp:6-31 (not virtual)
+  void set_typ_vals_ode(const std::vector<amrex::Real>& v)
+  {
+    if ((int)v.size() != NUM_SPECIES + 1) throw std::runtime_error("typ_vals size");
+    for (int i = 0; i < NUM_SPECIES + 1; ++i) m_typ_vals[i] = v[i];
+  }
+
  protected:
   int verbose{0};
+  amrex::GpuArray<amrex::Real, NUM_SPECIES + 1> m_typ_vals = {{0.0}};
 };
 
 }  // namespace reactions

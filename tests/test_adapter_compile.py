@@ -1,8 +1,8 @@
 """The C++ adapter include/ReactorHIP.H compiled (g++) against the ReactorBase
 interface restated in tests/adapter/ReactorBase.H (Source/Reactions/
 ReactorBase.H:14-106, Source/Factory.H): every override must match the
-reference signatures, the four *_HIP names (and, with PP_CHEM_REFERENCE_NAMES,
-ReactorBDF / ReactorRK64) are in the factory table from static
+reference signatures, the five *_HIP names (and, with PP_CHEM_REFERENCE_NAMES,
+ReactorBDF / ReactorRK64 / ReactorCvode) are in the factory table from static
 initialization, and init reports the reference's abort message for a bad
 reactor_type and an error for an unknown mechanism.  Runs without a GPU: the
 library rejects these configurations before any device call."""
@@ -35,9 +35,10 @@ def test_adapter_compiles_registers_and_aborts(tmp_path, ref_names):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr[-2000:]
     names = {l.split()[1] for l in out.stdout.splitlines() if l.startswith("name ")}
-    want = {"ReactorBDF_HIP", "ReactorRK64_HIP", "ReactorBDFAdaptive_HIP", "ReactorBDFBatched_HIP"}
+    want = {"ReactorBDF_HIP", "ReactorRK64_HIP", "ReactorBDFAdaptive_HIP", "ReactorBDFBatched_HIP",
+            "ReactorCvode_HIP"}
     if ref_names:
-        want |= {"ReactorBDF", "ReactorRK64"}
+        want |= {"ReactorBDF", "ReactorRK64", "ReactorCvode"}
     assert names == want
     assert "abort reactor_type: Wrong reactor type. Choose between 1 (e) or 2 (h)." in out.stdout
     assert "abort mechanism: ReactorHIP: mechanism not compiled" in out.stdout

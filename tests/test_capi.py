@@ -60,5 +60,6 @@ def test_mechanism_list_and_factory_names():
     from pelephysics_amd.reactor import mechanisms, ReactorBase, ReactorError
     assert set(mechanisms()) >= {"LiDryer", "drm19"}
     with pytest.raises(ReactorError):
-        ReactorBase.create("ReactorCvode")
+        ReactorBase.create("ReactorArkode")  # not provided (Factory.H:135-147 aborts likewise)
     assert ReactorBase.create("ReactorBDF").identifier == "ReactorBDF"
+    assert ReactorBase.create("ReactorCvode").identifier == "ReactorCvode"

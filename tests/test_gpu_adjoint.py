@@ -139,7 +139,8 @@ def test_adjoint_per_parameter_64_cells(models, oracles):
       / non-FMA oracle spread plus 64 ulp(o) / delta), and at least 95% of
       the Y_CO and Y_OH sensitivities above 1e-3 of their cell's largest are;
     * delta = 1e-6: resolved where 1e-4 |fd_j| exceeds that quotient's
-      measured error |fd(1e-6) - fd(1e-4)| plus its FMA spread (at 1e-6 the
+      measured error |fd(1e-6) - fd(1e-4)| plus its FMA spread and 64 ulp(o)
+      / delta (at 1e-6 the
       quotients of outputs near 1e-12, such as Y_CO in cold cells, carry
       errors of a few 1e-4 from the integration's rounding).
 
@@ -175,7 +176,7 @@ def test_adjoint_per_parameter_64_cells(models, oracles):
     fd4, fd4b = quotients(O, 1e-4), quotients(O2, 1e-4)
     fd6, fd6b = quotients(O, DELTA), quotients(O2, DELTA)
     noise4 = 4 * np.abs(fd4 - fd4b) + (64 * np.spacing(np.abs(o0)) / 1e-4)[:, None, :]
-    noise6 = np.abs(fd6 - fd4) + 4 * np.abs(fd6 - fd6b)
+    noise6 = np.abs(fd6 - fd4) + 4 * np.abs(fd6 - fd6b) + (64 * np.spacing(np.abs(o0)) / DELTA)[:, None, :]
     for fd, noise, tag in ((fd4, noise4, "1e-4"), (fd6, noise6, "1e-6")):
         resolved = 1e-4 * np.abs(fd) > noise
         err = np.abs(grad - fd)

@@ -221,3 +221,73 @@ def test_adaptive_accuracy_against_radau(models, oracles):
         assert abs(y[-1] - ref[-1]) <= 1e-6 * ref[-1]
         Y, Yr = y[:-1] / y[:-1].sum(), ref[:-1] / ref[:-1].sum()
         assert (np.abs(Y - Yr) <= 1e-6 * np.abs(Yr) + 1e-10).all()
+
+
+def _scipy_bdf_with_order_cap(cap):
+    """SciPy's BDF with its order limit at cap: bdf.py's source (read from
+    the installed SciPy) with `if order < MAX_ORDER:` -- the order-raising
+    test of the step-size selection -- reading `order < CAP`, loaded as a
+    module of its package; the arrays keep MAX_ORDER's size."""
+    import inspect
+    import types
+    import scipy.integrate._ivp.bdf as bdf
+    src = inspect.getsource(bdf)
+    assert src.count("if order < MAX_ORDER:") == 1
+    mod = types.ModuleType("scipy.integrate._ivp.bdf_order_cap")
+    mod.__package__ = "scipy.integrate._ivp"
+    mod.CAP = cap
+    exec(compile(src.replace("if order < MAX_ORDER:", "if order < CAP:"), mod.__name__, "exec"), mod.__dict__)
+    return mod.BDF
+
+
+@pytest.mark.parametrize("cap", [2, 3])
+def test_adaptive_order_cap_matches_scipy(cap):
+    """max_order (ReactorCvode's cvode.max_order) against SciPy's BDF with the
+    same cap: identical step, RHS and LU counts and results."""
+    A, b, y0 = _linear_system(8, seed=2)
+    r, y, st = oracle.adaptive_linear(A, y0, 1e-2, b=b, rtol=1e-9, atol=1e-12, max_order=cap)
+    sol = solve_ivp(lambda t, x: A @ x + b, (0, 1e-2), y0, method=_scipy_bdf_with_order_cap(cap), rtol=1e-9,
+                    atol=1e-12, jac=lambda t, x: A)
+    assert st["nsteps"] == len(sol.t) - 1 and st["nrhs"] == sol.nfev and st["nlu"] == sol.nlu
+    np.testing.assert_allclose(y, sol.y[:, -1], rtol=1e-11, atol=1e-14)
+    _, _, full = oracle.adaptive_linear(A, y0, 1e-2, b=b, rtol=1e-9, atol=1e-12)
+    assert st["nsteps"] > full["nsteps"]  # the cap binds
+
+
+def test_adaptive_newton_iterations_match_scipy(monkeypatch):
+    """newton_maxiter (ode.max_nls_iters) against SciPy's BDF with
+    NEWTON_MAXITER set to the same value."""
+    import scipy.integrate._ivp.bdf as bdf
+    A, b, y0 = _linear_system(8, seed=2)
+    monkeypatch.setattr(bdf, "NEWTON_MAXITER", 3)
+    sol = solve_ivp(lambda t, x: A @ x + b, (0, 1e-2), y0, method="BDF", rtol=1e-9, atol=1e-12,
+                    jac=lambda t, x: A)
+    r, y, st = oracle.adaptive_linear(A, y0, 1e-2, b=b, rtol=1e-9, atol=1e-12, newton_maxiter=3)
+    assert st["nsteps"] == len(sol.t) - 1 and st["nrhs"] == sol.nfev and st["nlu"] == sol.nlu
+    np.testing.assert_allclose(y, sol.y[:, -1], rtol=1e-11, atol=1e-14)
+
+
+def test_adaptive_vector_atol_matches_scipy():
+    """Per-component absolute tolerances (ReactorCvode's atol x typical
+    values) against SciPy's BDF with the same atol vector."""
+    A, b, y0 = _linear_system(8, seed=2)
+    av = 1e-12 * np.linspace(0.5, 30.0, 8)
+    r, y, st = oracle.adaptive_linear(A, y0, 1e-2, b=b, rtol=1e-9, atol=1e-12, atolv=av)
+    sol = solve_ivp(lambda t, x: A @ x + b, (0, 1e-2), y0, method="BDF", rtol=1e-9, atol=av,
+                    jac=lambda t, x: A)
+    assert st["nsteps"] == len(sol.t) - 1 and st["nrhs"] == sol.nfev and st["nlu"] == sol.nlu
+    np.testing.assert_allclose(y, sol.y[:, -1], rtol=1e-11, atol=1e-14)
+
+
+@pytest.mark.parametrize("age", [1, 3, 20])
+def test_adaptive_jacobian_age(age):
+    """jac_age (the Jacobian re-evaluated once it is `age` steps old): at
+    least one Jacobian per `age` steps, and the result within the tolerance
+    of the exact solution of the linear system (no SciPy counterpart: parity
+    of the counts is pinned by this bound only)."""
+    A, b, y0 = _linear_system(8, seed=2)
+    r, y, st = oracle.adaptive_linear(A, y0, 1e-2, b=b, rtol=1e-9, atol=1e-12, jac_age=age)
+    assert st["njac"] >= st["nsteps"] // age
+    sol = solve_ivp(lambda t, x: A @ x + b, (0, 1e-2), y0, method="Radau", rtol=1e-13, atol=1e-16,
+                    jac=lambda t, x: A)
+    np.testing.assert_allclose(y, sol.y[:, -1], rtol=1e-6, atol=1e-9)
