@@ -141,6 +141,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, OC
     cid = q.order ? q.order[cell] : cell;
     io.template load<M>(cid, yv, src, rhoe, rhoesrc, P.reactor_type);
     for (int i = 0; i < n; ++i) D[i] = yv[i];
+    // the higher rows are written before they are read; zeroed so that no
+    // path reads indeterminate values or a previous cell's
+    for (int i = n; i < (AMAXORD + 3) * n; ++i) D[i] = 0.0;
     t = 0.0;
     phase = PH_F0;
     order = 1;

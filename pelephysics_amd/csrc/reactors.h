@@ -480,6 +480,9 @@ PPC_DI int adaptive_cell(const AdaptParamsDev& p, const int reactor_type, double
     D[i] = y0[i];
     D[n + i] = f[i] * h_abs;
   }
+  // the higher rows are written before they are read; zeroed so that no
+  // path reads indeterminate values
+  for (int i = 2 * n; i < (AMAXORD + 3) * n; ++i) D[i] = 0.0;
   int order = 1, n_equal = 0;
   bool lu_valid = false;
   bdf_jacobian<M>(reactor_type, y0, 1.0, J, J);
