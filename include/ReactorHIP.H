@@ -14,14 +14,22 @@
 //                                defaults on the FP64 adaptive BDF; cvode.max_order,
 //                                cvode.max_substeps, ode.max_nls_iters and the
 //                                typical values of set_typ_vals_ode)
+//   "ReactorArkode_HIP"       -> PP_CHEM_ARKODE        (ReactorArkode's explicit RK
+//                                tables and step controllers: ode.rk_method,
+//                                ode.rk_controller, the typical values)
 // The ode.* ParmParse keys of ReactorBDF::init / ReactorRK64::init are read
 // with the same names; ode.rtol / ode.atol / ode.max_steps for the adaptive
 // forms.  The mechanism name comes from ode.mechanism (default: the string of
 // the compiled PelePhysics mechanism passed as PP_CHEM_MECHANISM).
+// In a Manifold build (USE_MANIFOLD_EOS: Eos_Model=Manifold, Chemistry_Model=
+// Null) the mechanism is "Manifold" with NUM_SPECIES - 1 parameters, and
+// ReactorRK64_HIP takes the tabulated model through set_eos_parm, as
+// ReactorRK64 does (ReactorRK64.H:123-128); the other integrators abort there
+// with the reference's message (ReactorBase.H:86-97).
 // With PP_CHEM_REFERENCE_NAMES defined (a build that leaves the reference's
 // ReactorBDF.cpp / ReactorRK64.cpp / ReactorCvode.cpp out of PelePhysics),
-// "ReactorBDF", "ReactorRK64" and "ReactorCvode" are registered too, so an
-// unchanged inputs file selects them.
+// "ReactorBDF", "ReactorRK64", "ReactorCvode" and "ReactorArkode" are
+// registered too, so an unchanged inputs file selects them.
 #pragma once
 
 #include <string>
@@ -42,7 +50,9 @@ public:
   {
     amrex::ParmParse pp("ode");
     std::string mech =
-#ifdef PP_CHEM_MECHANISM
+#if defined(USE_MANIFOLD_EOS)
+      "Manifold";
+#elif defined(PP_CHEM_MECHANISM)
       PP_CHEM_MECHANISM;
 #else
       "drm19";
@@ -50,6 +60,9 @@ public:
     pp.query("mechanism", mech);
     if (pp_chem_config_init(&m_cfg, mech.c_str(), MODE, reactor_type) != 0) {
       amrex::Abort("ReactorHIP: mechanism not compiled into libpele_chem_hip");
+    }
+    if (mech == "Manifold") {
+      m_cfg.manifold_dim = NUM_SPECIES - 1;  // Mechanisms/Null/mechanism.H:11
     }
     pp.query("verbose", this->verbose);
     pp.query("bdf_nonlinear_iters", m_cfg.bdf_nonlinear_iters);
@@ -80,6 +93,11 @@ public:
       amrex::ParmParse ppcv("cvode");
       ppcv.query("max_order", m_cfg.cvode_max_order);
       ppcv.query("max_substeps", m_cfg.max_steps);
+    }
+    if constexpr (MODE == PP_CHEM_ARKODE) {
+      // ReactorArkode.cpp:12-18
+      pp.query("rk_method", m_cfg.erk_method);
+      pp.query("rk_controller", m_cfg.erk_controller);
     }
     bool clean = false;
     pp.query("clean_init_massfrac", clean);  // ReactorBDF.cpp:127, ReactorRK64.cpp:18
@@ -230,6 +248,49 @@ public:
     }
   }
 
+#ifdef USE_MANIFOLD_EOS
+  // ReactorRK64::set_eos_parm (ReactorRK64.H:123-128) with the tabulated
+  // manifold model: the parameter block is PeleParams' device copy and the
+  // table's arrays are pinned host memory (Table.H:47-139), so the block and
+  // the table header are copied to the host and the arrays handed over as
+  // they are.
+  void set_eos_parm(
+    const pele::physics::eos::EosParm<pele::physics::PhysicsType::eos_type>* eosparm) override
+  {
+    if constexpr (MODE != PP_CHEM_RK64) {
+      amrex::Abort("Manifold EOS only supported with ReactorRK64 and ReactorNull for now");
+    } else {
+      pele::physics::eos::EosParm<pele::physics::eos::Manifold> ep;
+      amrex::Gpu::dtoh_memcpy(&ep, eosparm, sizeof(ep));
+      pele::physics::BlackBoxFunctionData bb;
+      amrex::Gpu::dtoh_memcpy(&bb, ep.manf_data, sizeof(bb));
+      if (bb.bbmodel != pele::physics::BlackBoxModel::TABLE) {
+        amrex::Abort("ReactorHIP: the Manifold EOS needs manifold.model = Table");
+      }
+      pele::physics::TabulatedFunctionData td;
+      amrex::Gpu::dtoh_memcpy(&td, ep.manf_data, sizeof(td));
+      pp_chem_manifold m{};
+      m.ndim = td.Ndim;
+      m.dim_lengths = td.dimLengths;
+      m.grids = td.grids;
+      m.nvar = td.Nvar;
+      m.values = td.values;
+      m.idx_density = ep.idx_density;
+      m.density_lookup = static_cast<int>(ep.dens_lookup);
+      m.idx_T = ep.idx_T;
+      m.compute_temperature = ep.compute_temperature ? 1 : 0;
+      for (int k = 0; k < 5; ++k) {
+        m.idx_src[k] = k < td.Ndim ? ep.idx_Wdot[k] : -1;
+      }
+      m.nominal_pressure = ep.Pnom_cgs;
+      if (pp_chem_set_manifold(m_h, &m) != 0) {
+        amrex::Abort(std::string("ReactorHIP: ") + pp_chem_last_error(m_h));
+      }
+      this->m_eosparm = eosparm;
+    }
+  }
+#endif
+
   ~ReactorHIP() override { close(); }
 
 private:
@@ -248,7 +309,7 @@ private:
   // a call whenever they changed
   void sync_typ_vals()
   {
-    if constexpr (MODE == PP_CHEM_CVODE) {
+    if constexpr (MODE == PP_CHEM_CVODE || MODE == PP_CHEM_ARKODE) {
       constexpr int n = NUM_SPECIES + 1;
       std::vector<double> v(n);
       for (int k = 0; k < n; ++k) {
@@ -292,12 +353,14 @@ inline constexpr char kReactorRK64_HIP[] = "ReactorRK64_HIP";
 inline constexpr char kReactorBDFAdaptive_HIP[] = "ReactorBDFAdaptive_HIP";
 inline constexpr char kReactorBDFBatched_HIP[] = "ReactorBDFBatched_HIP";
 inline constexpr char kReactorCvode_HIP[] = "ReactorCvode_HIP";
+inline constexpr char kReactorArkode_HIP[] = "ReactorArkode_HIP";
 
 template class ReactorHIP<PP_CHEM_BDF, kReactorBDF_HIP>;
 template class ReactorHIP<PP_CHEM_RK64, kReactorRK64_HIP>;
 template class ReactorHIP<PP_CHEM_BDF_ADAPTIVE, kReactorBDFAdaptive_HIP>;
 template class ReactorHIP<PP_CHEM_BDF_ADAPTIVE_MIXED, kReactorBDFBatched_HIP>;
 template class ReactorHIP<PP_CHEM_CVODE, kReactorCvode_HIP>;
+template class ReactorHIP<PP_CHEM_ARKODE, kReactorArkode_HIP>;
 
 // Odr-use of each registration flag: the names are in the factory table from
 // static initialization on (explicit instantiation alone does not define the
@@ -307,19 +370,23 @@ inline const bool kReactorHIPRegistered =
   ReactorBase::Register<ReactorHIP<PP_CHEM_RK64, kReactorRK64_HIP>>::registered &&
   ReactorBase::Register<ReactorHIP<PP_CHEM_BDF_ADAPTIVE, kReactorBDFAdaptive_HIP>>::registered &&
   ReactorBase::Register<ReactorHIP<PP_CHEM_BDF_ADAPTIVE_MIXED, kReactorBDFBatched_HIP>>::registered &&
-  ReactorBase::Register<ReactorHIP<PP_CHEM_CVODE, kReactorCvode_HIP>>::registered;
+  ReactorBase::Register<ReactorHIP<PP_CHEM_CVODE, kReactorCvode_HIP>>::registered &&
+  ReactorBase::Register<ReactorHIP<PP_CHEM_ARKODE, kReactorArkode_HIP>>::registered;
 
 #ifdef PP_CHEM_REFERENCE_NAMES
 inline constexpr char kReactorBDF_ref[] = "ReactorBDF";
 inline constexpr char kReactorRK64_ref[] = "ReactorRK64";
 inline constexpr char kReactorCvode_ref[] = "ReactorCvode";
+inline constexpr char kReactorArkode_ref[] = "ReactorArkode";
 template class ReactorHIP<PP_CHEM_BDF, kReactorBDF_ref>;
 template class ReactorHIP<PP_CHEM_RK64, kReactorRK64_ref>;
 template class ReactorHIP<PP_CHEM_CVODE, kReactorCvode_ref>;
+template class ReactorHIP<PP_CHEM_ARKODE, kReactorArkode_ref>;
 inline const bool kReactorHIPReferenceNames =
   ReactorBase::Register<ReactorHIP<PP_CHEM_BDF, kReactorBDF_ref>>::registered &&
   ReactorBase::Register<ReactorHIP<PP_CHEM_RK64, kReactorRK64_ref>>::registered &&
-  ReactorBase::Register<ReactorHIP<PP_CHEM_CVODE, kReactorCvode_ref>>::registered;
+  ReactorBase::Register<ReactorHIP<PP_CHEM_CVODE, kReactorCvode_ref>>::registered &&
+  ReactorBase::Register<ReactorHIP<PP_CHEM_ARKODE, kReactorArkode_ref>>::registered;
 #endif
 
 } // namespace pele::physics::reactions

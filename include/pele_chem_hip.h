@@ -39,13 +39,21 @@ enum {
   PP_CHEM_BDF_ADAPTIVE_MIXED = 3,  /* batched adaptive BDF: persistent work queue, FP32 Newton-matrix
                                       LU, FP64 residuals, rates and error control */
   PP_CHEM_BDF_BATCHED_FP64 = 4,    /* batched adaptive BDF with an FP64 LU */
-  PP_CHEM_CVODE = 5                /* "ReactorCvode": the FP64 adaptive BDF (one lane per cell) with
+  PP_CHEM_CVODE = 5,               /* "ReactorCvode": the FP64 adaptive BDF (one lane per cell) with
                                       ReactorCvode's options and defaults (rtol 1e-6, atol 1e-10 x the
                                       typical values, order <= cvode.max_order = 2, ode.max_nls_iters
                                       = 3 Newton iterations, Jacobian age <= 100 steps,
                                       cvode.max_substeps = 10000); FC and the return value count RHS
                                       evaluations (ReactorCvode.cpp:1365-1372).  Not SUNDIALS' CVODE:
                                       the same option set on this library's NDF/BDF integrator. */
+  PP_CHEM_ARKODE = 6               /* "ReactorArkode": explicit Runge-Kutta with the table of
+                                      ode.rk_method (erk_method; ReactorArkode.cpp:26-83, all but codes
+                                      43 and 54) and the error-based step control of ode.rk_controller
+                                      (erk_controller: 0 PID, 1 PI, 2 I, 3 explicit Gustafsson), rtol /
+                                      atol x typical values, one cell per lane with its own step size;
+                                      FC and the return value count RHS evaluations
+                                      (ReactorArkode.cpp:239-260).  Not SUNDIALS' ARKODE: its tables and
+                                      published adaptivity on this library's stepping. */
 };
 
 enum {
@@ -57,7 +65,7 @@ enum {
 };
 
 typedef struct {
-  const char* mechanism; /* "LiDryer", "drm19", ... */
+  const char* mechanism; /* "LiDryer", "drm19", ...; "Manifold": the Manifold EOS (below) */
   int reactor_type;      /* 1: e (constant volume, rho e), 2: h (rho h) */
   int mode;              /* PP_CHEM_* integrator */
   int mod_reactor;       /* 1: time += dt_react on return (-DMOD_REACTOR) */
@@ -112,6 +120,17 @@ typedef struct {
    * it is re-evaluated at the predicted state (CVodeSetJacEvalFrequency; 0:
    * only when Newton fails to converge). */
   int cvode_max_order, max_nls_iters, jac_max_age;
+  /* mechanism "Manifold" (the reference's Eos_Model=Manifold with
+   * Chemistry_Model=Null, Mechanisms/Null/mechanism.H:11): the number of
+   * manifold parameters D = MANIFOLD_DIM (1..5; pp_chem_config_init sets 1),
+   * NUM_SPECIES = D + 1 (the parameters times rho, then rho).  ReactorRK64
+   * only, as the reference propagates the Manifold EOS through ReactorRK64
+   * alone (ReactorBase.H:84-97, ReactorRK64.H:123-128). */
+  int manifold_dim;
+  /* PP_CHEM_ARKODE (0 elsewhere): ode.rk_method (pp_chem_config_init: 40,
+   * ARKODE_ZONNEVELD_5_3_4; unknown codes select it, as the reference does;
+   * 43 and 54 are refused) and ode.rk_controller (0 PID). */
+  int erk_method, erk_controller;
 } pp_chem_config;
 
 /* amrex::Array4<Real> layout: element (i,j,k,n) is at
@@ -138,6 +157,33 @@ typedef struct pp_chem pp_chem;
  * (typ_vals[0] <= 0: the scalar atol again).  The other integrators ignore
  * them, as the reference's ReactorBDF and ReactorRK64 do. */
 int pp_chem_set_typ_vals(pp_chem* h, const double* typ_vals, int n);
+
+/* The Manifold EOS's parameters with a tabulated model, the data of
+ * ReactorRK64::set_eos_parm (ReactorRK64.H:123-128): EosParm<Manifold>
+ * (Source/Eos/EosParams.H:39-49) over TabulatedFunctionData
+ * (Source/Utility/BlackBoxFunction/Table.H:11-17, the .ctb file layout of
+ * read_table, Table.H:63-139).  Host arrays, copied by pp_chem_set_manifold. */
+typedef struct {
+  int ndim;                /* table dimensions = manifold parameters (must equal manifold_dim) */
+  const int* dim_lengths;  /* ndim grid lengths, each >= 2 */
+  const double* grids;     /* the ndim grids, concatenated, each strictly ascending */
+  int nvar;                /* tabulated variables */
+  const double* values;    /* nvar x prod(dim_lengths), first dimension fastest */
+  int idx_density;         /* variable holding RHO (density_lookup 0), lnRHO (1) or invRHO (2) */
+  int density_lookup;
+  int idx_T;               /* variable holding T */
+  int compute_temperature; /* manifold.compute_temperature: 1: T from the table, 0: T = 1 */
+  int idx_src[5];          /* source of each manifold parameter (SRC_<name>), -1: none (zero) */
+  double nominal_pressure; /* manifold.nominal_pressure_cgs */
+} pp_chem_manifold;
+
+/* ReactorRK64::set_eos_parm for the Manifold EOS: the table every lookup of
+ * the next calls interpolates in (multilinear; indices clamped to the grid,
+ * weights to [0, 1], Table.H:183-219,470-545).  A handle of mechanism
+ * "Manifold" returns PP_CHEM_EINVAL from its react calls until this is set;
+ * other mechanisms refuse it.  flatten / unflatten, the adjoint and the
+ * mechanism functions are not available with the Manifold EOS. */
+int pp_chem_set_manifold(pp_chem* h, const pp_chem_manifold* m);
 
 /* Fill cfg with the reference defaults for the given mechanism and mode. */
 int pp_chem_config_init(pp_chem_config* cfg, const char* mechanism, int mode, int reactor_type);

@@ -824,19 +824,24 @@ void ppo_chem_jac(const ppo_mech* m, int reactor_type, const double* y, double* 
 
 /* ---------------------------------------------------- generic systems -- */
 typedef struct {
-  int n, kind; /* 0 chemistry, 1 linear */
+  int n, kind; /* 0 chemistry, 1 linear, 2 manifold */
   const ppo_mech* m;
   int reactor_type;
   double rhoe_init, rhoesrc;
   const double* rYsrc;
   const double *A, *b;
   int nrhs, njac;
+  const ppo_manifold* mf;
 } sys_t;
+
+static void manifold_rhs(const sys_t* s, const double* y, double* f);
 
 static void sys_rhs(sys_t* s, double trel, const double* y, double* f) {
   s->nrhs++;
   if (s->kind == 0) {
     ppo_fkernelspec(s->m, trel, s->reactor_type, y, f, s->rhoe_init, s->rhoesrc, s->rYsrc);
+  } else if (s->kind == 2) {
+    manifold_rhs(s, y, f);
   } else {
     for (int i = 0; i < s->n; ++i) {
       double v = s->b ? s->b[i] : 0.0;
@@ -1438,7 +1443,7 @@ int ppo_react_bdf_1d(const ppo_mech* m, const ppo_bdf_params* p, int reactor_typ
     bdf_ws_init(&w, n);
 #pragma omp for schedule(dynamic, 64)
     for (int c = 0; c < ncells; ++c) {
-      sys_t s = {n, 0, m, reactor_type, rX[c], rXsrc[c], ysrc + (size_t)c * m->ns, NULL, NULL, 0, 0};
+      sys_t s = {n, 0, m, reactor_type, rX[c], rXsrc[c], ysrc + (size_t)c * m->ns, NULL, NULL, 0, 0, NULL};
       int r = bdf_core(&s, p, y + (size_t)c * n, dt_react, &w);
       if (r < 0) bad++;
       else total += r;
@@ -1462,7 +1467,7 @@ int ppo_react_rk64_1d(const ppo_mech* m, const ppo_rk64_params* p, int reactor_t
     double* w = (double*)malloc(sizeof(double) * 3 * n);
 #pragma omp for schedule(dynamic, 64)
     for (int c = 0; c < ncells; ++c) {
-      sys_t s = {n, 0, m, reactor_type, rX[c], rXsrc[c], ysrc + (size_t)c * m->ns, NULL, NULL, 0, 0};
+      sys_t s = {n, 0, m, reactor_type, rX[c], rXsrc[c], ysrc + (size_t)c * m->ns, NULL, NULL, 0, 0, NULL};
       int r = rk64_core(&s, p, y + (size_t)c * n, dt_react, w);
       total += r;
       if (FC) FC[c] = r;
@@ -1489,7 +1494,7 @@ int ppo_react_adaptive_1d(const ppo_mech* m, const ppo_adapt_params* p, int reac
     ppo_adapt_stats loc = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma omp for schedule(dynamic, 64)
     for (int c = 0; c < ncells; ++c) {
-      sys_t s = {n, 0, m, reactor_type, rX[c], rXsrc[c], ysrc + (size_t)c * m->ns, NULL, NULL, 0, 0};
+      sys_t s = {n, 0, m, reactor_type, rX[c], rXsrc[c], ysrc + (size_t)c * m->ns, NULL, NULL, 0, 0, NULL};
       ppo_adapt_stats one = {0, 0, 0, 0, 0, 0, 0, 0};
       int r = adaptive_core(&s, p, y + (size_t)c * n, dt_react, &w, &one);
       if (r < 0) bad++;
@@ -1541,7 +1546,7 @@ int ppo_react_box(const ppo_mech* m, int mode, const ppo_bdf_params* pb,
       temp = (reactor_type == 1) ? ppo_T_given_ey(m, enrg, Y, temp, NULL)
                                  : ppo_T_given_hy(m, enrg, Y, temp, NULL);
       y[ns] = temp;
-      sys_t s = {n, 0, m, reactor_type, rE[c], rEsrc[c], src, NULL, NULL, 0, 0};
+      sys_t s = {n, 0, m, reactor_type, rE[c], rEsrc[c], src, NULL, NULL, 0, 0, NULL};
       int r;
       if (mode == 0) r = bdf_core(&s, pb, y, dt_react, &wb);
       else if (mode == 1) r = rk64_core(&s, pr, y, dt_react, rw);
@@ -1571,9 +1576,312 @@ int ppo_react_box(const ppo_mech* m, int mode, const ppo_bdf_params* pb,
   return (int)(total / (double)ncells);
 }
 
+/* ------------------------------------------- Manifold EOS (tabulated) -- */
+/* Left node of x on the grid g[0..n-1]: the first node >= x among g[1..n-2],
+ * minus one, kept in [0, n-2] (the host search of Table.H:513-520); the weight
+ * of the left node is (g[i+1] - x) / (g[i+1] - g[i]) clamped to [0, 1]. */
+static void mf_bracket(const double* g, int n, double x, int* idx, double* w) {
+  int lo = 1, hi = n - 1; /* search [1, n-2] for the first g >= x; hi = n-1 means none */
+  while (lo < hi) {
+    int mid = lo + (hi - lo) / 2;
+    if (g[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  int i = lo - 1;
+  if (i < 0) i = 0;
+  if (i > n - 2) i = n - 2;
+  *idx = i;
+  double inv = 1.0 / (g[i + 1] - g[i]);
+  double a = (g[i + 1] - x) * inv;
+  *w = a < 0.0 ? 0.0 : (a > 1.0 ? 1.0 : a);
+}
+
+/* Value on the slab of dimensions 0..d-1 starting at data (stride[k] apart
+ * along dimension k): the weighted pair along the highest dimension, each a
+ * value of the lower slab (Table.H:183-219). */
+static double mf_slab(int d, const int* idx, const double* w, const int* stride, const double* data) {
+  if (d == 1) return w[0] * data[idx[0]] + (1.0 - w[0]) * data[idx[0] + 1];
+  const double* lo = data + (size_t)idx[d - 1] * stride[d - 1];
+  const double* hi = lo + stride[d - 1];
+  double a = mf_slab(d - 1, idx, w, stride, lo);
+  double b = mf_slab(d - 1, idx, w, stride, hi);
+  return w[d - 1] * a + (1.0 - w[d - 1]) * b;
+}
+
+static void mf_point(const ppo_manifold* t, const double* x, int* idx, double* w, int* stride, size_t* nper) {
+  const double* g = t->grids;
+  size_t st = 1;
+  for (int k = 0; k < t->ndim; ++k) {
+    mf_bracket(g, t->dim_len[k], x[k], &idx[k], &w[k]);
+    stride[k] = (int)st;
+    st *= (size_t)t->dim_len[k];
+    g += t->dim_len[k];
+  }
+  *nper = st;
+}
+
+double ppo_manifold_value(const ppo_manifold* t, int ivar, const double* x) {
+  int idx[5], stride[5];
+  double w[5];
+  size_t nper;
+  mf_point(t, x, idx, w, stride, &nper);
+  return mf_slab(t->ndim, idx, w, stride, t->values + (size_t)ivar * nper);
+}
+
+double ppo_manifold_density(const ppo_manifold* t, const double* Y) {
+  double r = ppo_manifold_value(t, t->idx_density, Y);
+  if (t->dens_lookup == 1) r = exp(r);
+  else if (t->dens_lookup == 2) r = 1.0 / r;
+  return r;
+}
+
+static double mf_temperature(const ppo_manifold* t, const double* Y) {
+  return t->compute_T ? ppo_manifold_value(t, t->idx_T, Y) : 1.0;
+}
+
+/* fKernelSpec under USE_MANIFOLD_EOS (ReactorUtils.H:358-415 with
+ * Manifold.H:39-52,265-276): Y = rhoY / rho with rho the last species, the
+ * parameter rates from the table (zero without a source), the density rate its
+ * external source, dT/dt = rhoE_src / (rho Cv) with Cv = 1 and zero species
+ * energies. */
+static void manifold_rhs(const sys_t* s, const double* y, double* f) {
+  const ppo_manifold* t = s->mf;
+  const int D = t->ndim;
+  double rho = y[D], rinv = 1.0 / rho, Y[5];
+  for (int k = 0; k < D; ++k) Y[k] = y[k] * rinv;
+  int idx[5], stride[5];
+  double w[5];
+  size_t nper;
+  mf_point(t, Y, idx, w, stride, &nper);
+  for (int k = 0; k < D; ++k) {
+    double src = t->idx_src[k] >= 0 ? mf_slab(D, idx, w, stride, t->values + (size_t)t->idx_src[k] * nper) : 0.0;
+    f[k] = src + s->rYsrc[k];
+  }
+  f[D] = s->rYsrc[D];
+  f[D + 1] = s->rhoesrc * rinv;
+}
+
+int ppo_manifold_rk64_1d(const ppo_manifold* t, const ppo_rk64_params* p, double* y, const double* ysrc,
+                         double* rX, const double* rXsrc, double dt_react, int ncells, int* FC) {
+  const int ns = t->ndim + 1, n = ns + 1;
+  double w[3 * 7];
+  long total = 0;
+  for (int c = 0; c < ncells; ++c) {
+    sys_t s = {n, 2, NULL, 2, rX[c], rXsrc[c], ysrc + (size_t)c * ns, NULL, NULL, 0, 0, t};
+    int r = rk64_core(&s, p, y + (size_t)c * n, dt_react, w);
+    total += r;
+    if (FC) FC[c] = r;
+    rX[c] = rX[c] + dt_react * rXsrc[c];
+  }
+  return ncells > 0 ? (int)(total / (double)ncells) : 0;
+}
+
+int ppo_manifold_rk64_box(const ppo_manifold* t, const ppo_rk64_params* p, double* rY, const double* rYsrc,
+                          double* T, double* rE, const double* rEsrc, double* FC, double dt_react, int ncells) {
+  const int D = t->ndim, ns = D + 1, n = ns + 1;
+  double w[3 * 7], y[7], src[6], Y[5];
+  long total = 0;
+  for (int c = 0; c < ncells; ++c) {
+    for (int k = 0; k < ns; ++k) {
+      y[k] = rY[(size_t)k * ncells + c];
+      src[k] = rYsrc[(size_t)k * ncells + c];
+    }
+    for (int k = 0; k < D; ++k) Y[k] = y[k] * (1.0 / y[D]);
+    y[ns] = mf_temperature(t, Y);
+    sys_t s = {n, 2, NULL, 2, rE[c], rEsrc[c], src, NULL, NULL, 0, 0, t};
+    int r = rk64_core(&s, p, y, dt_react, w);
+    total += r;
+    for (int k = 0; k < ns; ++k) rY[(size_t)k * ncells + c] = y[k];
+    for (int k = 0; k < D; ++k) Y[k] = y[k] * (1.0 / y[D]);
+    rE[c] = rE[c] + dt_react * rEsrc[c];
+    T[c] = mf_temperature(t, Y);
+    if (FC) FC[c] = (double)r;
+  }
+  return ncells > 0 ? (int)(total / (double)ncells) : 0;
+}
+
+/* ------------------------------------------------- ReactorArkode (ERK) -- */
+#define ERK_SAFETY 0.96
+#define ERK_BIAS 1.5
+#define ERK_GROWTH 20.0
+#define ERK_ETAMX1 10000.0
+#define ERK_ETAMXF 0.3
+#define ERK_ETAMIN 0.1
+#define ERK_SMALL_NEF 2
+#define ERK_FIXED_LB 1.0
+#define ERK_FIXED_UB 1.5
+
+/* Step factor the controller asks for from the scaled error estimates
+ * e1 (this step), e2, e3 (the two accepted steps before) of a method whose
+ * error estimate is of order k = p + 1. */
+static double erk_controller(int kind, double k, double e1, double e2, double e3) {
+  switch (kind) {
+    case 1: /* PI */
+      return pow(e1, -0.8 / k) * pow(e2, 0.31 / k);
+    case 2: /* I */
+      return pow(e1, -1.0 / k);
+    case 3: /* explicit Gustafsson: e1^(-k1/k) (e1/e2)^(-k2/k), I on the first step */
+      return e2 > 0 ? pow(e1, -0.367 / k) * pow(e1 / e2, -0.268 / k) : pow(e1, -1.0 / k);
+    default: /* PID */
+      return pow(e1, -0.58 / k) * pow(e2, 0.21 / k) * pow(e3, -0.1 / k);
+  }
+}
+
+static double erk_wrms(int n, const double* v, const double* wt) {
+  double acc = 0.0;
+  for (int i = 0; i < n; ++i) acc += (v[i] * wt[i]) * (v[i] * wt[i]);
+  return sqrt(acc / n);
+}
+
+/* One cell from t = 0 to t_end; w holds (s + 4) n doubles. */
+static int erk_core(sys_t* sy, const ppo_erk_table* T, const ppo_erk_params* p, double* y, double t_end,
+                    double* w, ppo_erk_stats* st) {
+  const int n = sy->n, S = T->s;
+  double *K = w, *yt = w + (size_t)S * n, *ynew = yt + n, *err = ynew + n, *wt = err + n;
+  int fsal = S > 1 && T->c[S - 1] == 1.0 && T->b[S - 1] == 0.0;
+  for (int j = 0; fsal && j < S - 1; ++j) fsal = T->A[(S - 1) * S + j] == T->b[j];
+  const double k = T->p + 1.0;
+  int nst = 0, nfe = 0, nrej = 0, nef = 0, after_fail = 0, first = 1, status = 0;
+  double t = 0.0, e2 = 1.0, e3 = 1.0, e2g = 0.0, last_dsm = 0.0, last_h = 0.0;
+#define ERK_ATOL(i) (p->atolv ? p->atolv[i] : p->atol)
+  /* f at the start: the first stage of the first step */
+  sys_rhs(sy, t, y, K);
+  nfe++;
+  int k0_valid = 1;
+  double h;
+  if (p->fixed_h > 0) {
+    h = p->fixed_h;
+  } else {
+    /* initial step (Hairer, Norsett & Wanner I, II.4) with the embedding's order */
+    for (int i = 0; i < n; ++i) wt[i] = 1.0 / (ERK_ATOL(i) + fabs(y[i]) * p->rtol);
+    double d0 = erk_wrms(n, y, wt), d1 = erk_wrms(n, K, wt);
+    double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+    h0 = fmin(h0, t_end);
+    for (int i = 0; i < n; ++i) yt[i] = y[i] + h0 * K[i];
+    sys_rhs(sy, t + h0, yt, ynew);
+    nfe++;
+    for (int i = 0; i < n; ++i) err[i] = ynew[i] - K[i];
+    double d2 = erk_wrms(n, err, wt) / h0;
+    double h1 = (d1 <= 1e-15 && d2 <= 1e-15) ? fmax(1e-6, h0 * 1e-3) : pow(0.01 / fmax(d1, d2), 1.0 / k);
+    h = fmin(fmin(100 * h0, h1), t_end);
+  }
+  while (t < t_end) {
+    if (p->max_steps > 0 && nst + nrej >= p->max_steps) { status = -2; break; }
+    const double min_step = 10 * fabs(nextafter(t, INFINITY) - t);
+    if (!(h >= min_step)) { status = -3; break; }
+    const double hh = fmin(h, t_end - t);
+    if (!k0_valid) {
+      sys_rhs(sy, t, y, K);
+      nfe++;
+    }
+    for (int i = 1; i < S; ++i) {
+      for (int m = 0; m < n; ++m) {
+        double acc = 0.0;
+        for (int j = 0; j < i; ++j) acc += T->A[i * S + j] * K[(size_t)j * n + m];
+        yt[m] = y[m] + hh * acc;
+      }
+      sys_rhs(sy, t + T->c[i] * hh, yt, K + (size_t)i * n);
+      nfe++;
+    }
+    for (int m = 0; m < n; ++m) {
+      double sb = 0.0, se = 0.0;
+      for (int j = 0; j < S; ++j) {
+        sb += T->b[j] * K[(size_t)j * n + m];
+        se += (T->b[j] - T->d[j]) * K[(size_t)j * n + m];
+      }
+      ynew[m] = y[m] + hh * sb;
+      err[m] = hh * se;
+    }
+    if (p->fixed_h > 0) {
+      memcpy(y, ynew, sizeof(double) * n);
+      t += hh;
+      nst++;
+      k0_valid = 0;
+      continue;
+    }
+    for (int i = 0; i < n; ++i) wt[i] = 1.0 / (ERK_ATOL(i) + fabs(y[i]) * p->rtol);
+    const double dsm = erk_wrms(n, err, wt);
+    const double e1 = fmax(ERK_BIAS * dsm, 1e-10);
+    double eta = ERK_SAFETY * erk_controller(p->controller, k, e1, p->controller == 3 ? e2g : e2, e3);
+    if (dsm <= 1.0) {
+      memcpy(y, ynew, sizeof(double) * n);
+      t += hh;
+      nst++;
+      if (hh == h) { /* the controller's own step, not the one clipped to t_end */
+        last_dsm = dsm;
+        last_h = hh;
+      }
+      if (fsal) memcpy(K, K + (size_t)(S - 1) * n, sizeof(double) * n);
+      k0_valid = fsal;
+      eta = fmin(eta, first ? ERK_ETAMX1 : ERK_GROWTH);
+      if (after_fail) eta = fmin(eta, 1.0);
+      if (eta >= ERK_FIXED_LB && eta <= ERK_FIXED_UB) eta = 1.0;
+      e3 = e2;
+      e2 = e1;
+      e2g = e1;
+      first = 0;
+      nef = 0;
+      after_fail = 0;
+      h = hh * eta;
+    } else {
+      nrej++;
+      nef++;
+      after_fail = 1;
+      eta = fmin(fmax(eta, ERK_ETAMIN), 1.0);
+      if (nef >= ERK_SMALL_NEF) eta = fmin(eta, ERK_ETAMXF);
+      k0_valid = 1; /* K0 is f(t, y), y unchanged */
+      h = hh * eta;
+    }
+  }
+#undef ERK_ATOL
+  if (st) {
+    st->nst = nst;
+    st->nfe = nfe;
+    st->nrej = nrej;
+    st->status = status;
+    st->last_dsm = last_dsm;
+    st->last_h = last_h;
+  }
+  return status < 0 ? status : nfe;
+}
+
+int ppo_react_erk_1d(const ppo_mech* m, const ppo_erk_table* T, const ppo_erk_params* p, int reactor_type,
+                     double* y, const double* ysrc, double* rX, const double* rXsrc, double dt_react,
+                     int ncells, ppo_erk_stats* stats, int nthreads) {
+  const int n = m->ns + 1;
+  long total = 0;
+  int bad = 0;
+#pragma omp parallel num_threads(threads_or_default(nthreads)) reduction(+ : total, bad)
+  {
+    double* w = (double*)malloc(sizeof(double) * (size_t)(T->s + 4) * n);
+#pragma omp for schedule(dynamic, 16)
+    for (int c = 0; c < ncells; ++c) {
+      sys_t s = {n, 0, m, reactor_type, rX[c], rXsrc[c], ysrc + (size_t)c * m->ns, NULL, NULL, 0, 0, NULL};
+      ppo_erk_stats one;
+      int r = erk_core(&s, T, p, y + (size_t)c * n, dt_react, w, &one);
+      if (stats) stats[c] = one;
+      if (r < 0) bad++;
+      else total += r;
+      rX[c] = rX[c] + dt_react * rXsrc[c];
+    }
+    free(w);
+  }
+  if (bad) return -1;
+  return ncells > 0 ? (int)(total / (double)ncells) : 0;
+}
+
+int ppo_erk_linear(int n, const double* A, const double* b, double* y, double t_end, const ppo_erk_table* T,
+                   const ppo_erk_params* p, ppo_erk_stats* st) {
+  sys_t s = {n, 1, NULL, 0, 0, 0, NULL, A, b, 0, 0, NULL};
+  double* w = (double*)malloc(sizeof(double) * (size_t)(T->s + 4) * n);
+  int r = erk_core(&s, T, p, y, t_end, w, st);
+  free(w);
+  return r;
+}
+
 int ppo_bdf_linear(int n, const double* A, const double* b, double* y, double dt_react,
                    const ppo_bdf_params* p) {
-  sys_t s = {n, 1, NULL, 0, 0, 0, NULL, A, b, 0, 0};
+  sys_t s = {n, 1, NULL, 0, 0, 0, NULL, A, b, 0, 0, NULL};
   bdf_ws w;
   bdf_ws_init(&w, n);
   int r = bdf_core(&s, p, y, dt_react, &w);
@@ -1583,7 +1891,7 @@ int ppo_bdf_linear(int n, const double* A, const double* b, double* y, double dt
 
 int ppo_rk64_linear(int n, const double* A, const double* b, double* y, double dt_react,
                     const ppo_rk64_params* p) {
-  sys_t s = {n, 1, NULL, 0, 0, 0, NULL, A, b, 0, 0};
+  sys_t s = {n, 1, NULL, 0, 0, 0, NULL, A, b, 0, 0, NULL};
   double* w = (double*)malloc(sizeof(double) * 3 * n);
   int r = rk64_core(&s, p, y, dt_react, w);
   free(w);
@@ -1592,7 +1900,7 @@ int ppo_rk64_linear(int n, const double* A, const double* b, double* y, double d
 
 int ppo_adaptive_linear(int n, const double* A, const double* b, double* y, double t_end,
                         const ppo_adapt_params* p, ppo_adapt_stats* st) {
-  sys_t s = {n, 1, NULL, 0, 0, 0, NULL, A, b, 0, 0};
+  sys_t s = {n, 1, NULL, 0, 0, 0, NULL, A, b, 0, 0, NULL};
   ad_ws w;
   ad_ws_init(&w, n);
   int r = adaptive_core(&s, p, y, t_end, &w, st);

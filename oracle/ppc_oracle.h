@@ -143,6 +143,66 @@ int ppo_react_box(const ppo_mech* m, int mode, const ppo_bdf_params* pb,
                   double* rE, const double* rEsrc, double* FC, double dt_react,
                   double time, int ncells, ppo_adapt_stats* stats, int nthreads);
 
+/* Manifold EOS with a tabulated model (Source/Eos/Manifold.H,
+ * Source/Utility/BlackBoxFunction/Table.H, Source/Eos/EosParams.H:39-49):
+ * ndim manifold parameters, NUM_SPECIES = ndim + 1 (rho last). */
+typedef struct {
+  int ndim, nvar;
+  int dim_len[5];
+  const double* grids;  /* concatenated ascending grids */
+  const double* values; /* nvar x prod(dim_len), first dimension fastest */
+  int idx_density, dens_lookup, idx_T, compute_T;
+  int idx_src[5];       /* -1: no source */
+} ppo_manifold;
+
+/* TabulatedFunction::get_value (Table.H:318-330): variable ivar at x[ndim]. */
+double ppo_manifold_value(const ppo_manifold* t, int ivar, const double* x);
+/* Manifold::PYT2R (Manifold.H:305-322): rho from the table at Y. */
+double ppo_manifold_density(const ppo_manifold* t, const double* Y);
+/* ReactorRK64 with the Manifold EOS: 1-D form (y[c*(ndim+2)+k], ysrc[c*(ndim+1)+k])
+ * and box form (SoA rY[k*ncells+c], T, rE, rEsrc, FC per cell). */
+int ppo_manifold_rk64_1d(const ppo_manifold* t, const ppo_rk64_params* p, double* y, const double* ysrc,
+                         double* rX, const double* rXsrc, double dt_react, int ncells, int* FC);
+int ppo_manifold_rk64_box(const ppo_manifold* t, const ppo_rk64_params* p, double* rY, const double* rYsrc,
+                          double* T, double* rE, const double* rEsrc, double* FC, double dt_react, int ncells);
+
+/* ReactorArkode: explicit Runge-Kutta with an embedded error estimate and
+ * step-size control (the ERK tables of ReactorArkode.cpp:26-83; ARKODE's
+ * published adaptivity: WRMS error test, Soderlind-type I / PI / PID /
+ * explicit-Gustafsson controllers with safety 0.96 and bias 1.5, growth at
+ * most 20 (10000 on the first step), reduction at least 0.1 and at most 0.3
+ * after 2 consecutive failures, no change for a factor in [1, 1.5]), one
+ * cell at a time. */
+typedef struct {
+  int s, q, p;       /* stages, order of b, order of the embedding d */
+  const double* c;   /* s */
+  const double* A;   /* s x s row-major, zero on and above the diagonal */
+  const double* b;   /* s */
+  const double* d;   /* s */
+} ppo_erk_table;
+
+typedef struct {
+  double rtol, atol;
+  const double* atolv; /* per-component absolute tolerances (n), or NULL */
+  int max_steps;       /* step attempts per call */
+  int controller;      /* 0 PID, 1 PI, 2 I, 3 explicit Gustafsson */
+  double fixed_h;      /* > 0: steps of this size without error control (tests) */
+} ppo_erk_params;
+
+typedef struct {
+  int nst, nfe, nrej, status; /* status < 0: the cell failed */
+  double last_dsm, last_h;    /* error norm and size of the last accepted step not clipped to t_end */
+} ppo_erk_stats;
+
+/* 1-D form (ReactorArkode.cpp:274-330 semantics: T from the vector; the
+ * return value and FC are RHS evaluations): returns int(average nfe) or -1
+ * when a cell failed; stats (may be NULL) receives per-cell counters. */
+int ppo_react_erk_1d(const ppo_mech* m, const ppo_erk_table* T, const ppo_erk_params* p, int reactor_type,
+                     double* y, const double* ysrc, double* rX, const double* rXsrc, double dt_react,
+                     int ncells, ppo_erk_stats* stats, int nthreads);
+int ppo_erk_linear(int n, const double* A, const double* b, double* y, double t_end, const ppo_erk_table* T,
+                   const ppo_erk_params* p, ppo_erk_stats* st);
+
 /* Linear test systems y' = A y + b (A row-major n x n) for closed-form checks. */
 int ppo_bdf_linear(int n, const double* A, const double* b, double* y, double dt_react,
                    const ppo_bdf_params* p);

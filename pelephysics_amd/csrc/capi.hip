@@ -13,12 +13,14 @@
 #include <vector>
 
 #include "../../include/pele_chem_hip.h"
+#include "erk_tables.h"
 #include "launch.h"
 
 namespace ppc {
 #define PPC_MECH(M) const MechLauncher* launcher_##M();
 #include "inst/registry.inc"
 #undef PPC_MECH
+const MechLauncher* manifold_launcher(int dim);  // manifold.hip
 }  // namespace ppc
 
 namespace {
@@ -34,6 +36,15 @@ const MechLauncher* find_mech(const char* name) {
   for (const MechLauncher* m : all)
     if (name && std::strcmp(m->name, name) == 0) return m;
   return nullptr;
+}
+
+// "Manifold": the Manifold EOS over a table of dim dimensions (the reference's
+// Eos_Model=Manifold with Chemistry_Model=Null and Manifold_Dim=dim).
+bool is_manifold_name(const char* name) { return name && std::strcmp(name, "Manifold") == 0; }
+
+const MechLauncher* resolve_mech(const char* name, int manifold_dim) {
+  if (is_manifold_name(name)) return ppc::manifold_launcher(manifold_dim);
+  return find_mech(name);
 }
 
 // Sums the per-cell counters and reduces min/max of the step counts.
@@ -197,6 +208,12 @@ struct pp_chem {
   unsigned char* scratch = nullptr;
   // ReactorCvode: absolute tolerances atol x typical value per component
   double* d_atolv = nullptr;
+  // Manifold EOS: the table's grids and values (device), set by
+  // pp_chem_set_manifold
+  bool manifold = false;
+  double* d_man = nullptr;
+  // ReactorArkode: the table (c, A, b, d) on the device
+  double* d_erk = nullptr;
 };
 
 namespace {
@@ -204,6 +221,20 @@ namespace {
 int fail(pp_chem* h, int code, const std::string& msg) {
   if (h) h->err = msg;
   return code;
+}
+
+// The Manifold EOS has nothing to evaluate before its table is set
+// (Manifold.H:333-338 refuses to construct without it).
+int manifold_unset(pp_chem* h) {
+  return h->manifold && !h->P.man.values
+             ? fail(h, PP_CHEM_EINVAL, "Manifold EOS must be initialized with table data (pp_chem_set_manifold)")
+             : 0;
+}
+
+// What the Manifold EOS does not provide (Manifold.H: no Jacobian, no
+// species thermodynamics; box_flatten constructs the EOS without its table).
+int manifold_refuses(pp_chem* h, const char* what) {
+  return h->manifold ? fail(h, PP_CHEM_EINVAL, std::string(what) + " is not available with the Manifold EOS") : 0;
 }
 
 int hip_fail(pp_chem* h, hipError_t e, const char* where) {
@@ -278,12 +309,17 @@ int finish_with(pp_chem* h, int ncells, hipStream_t s, float kernel_ms, const pp
   if (ncells == 0) return 0;
   // ReactorCvode reports RHS evaluations (CVodeGetNumRhsEvals,
   // ReactorCvode.cpp:1365-1372), the other integrators steps
-  return (int)((double)(h->cfg.mode == PP_CHEM_CVODE ? S.nrhs : S.nsteps) / (double)ncells);
+  // ReactorCvode and ReactorArkode report RHS evaluations (ReactorCvode.cpp:1365-1372,
+  // ReactorArkode.cpp:239-260)
+  const bool rhs_count = h->cfg.mode == PP_CHEM_CVODE || h->cfg.mode == PP_CHEM_ARKODE;
+  return (int)((double)(rhs_count ? S.nrhs : S.nsteps) / (double)ncells);
 }
 
 bool is_batched(int mode) { return mode == ppc::MODE_BATCHED_MIXED || mode == ppc::MODE_BATCHED_FP64; }
 // Kernels of a mode: ReactorCvode runs the plain FP64 adaptive BDF with its options.
-int kmode(int mode) { return mode == PP_CHEM_CVODE ? ppc::MODE_ADAPTIVE : mode; }
+int kmode(int mode) {
+  return mode == PP_CHEM_CVODE ? ppc::MODE_ADAPTIVE : (mode == PP_CHEM_ARKODE ? ppc::MODE_ERK : mode);
+}
 
 // ---- device scratch of pp_chem_react_1d_noalloc: one array per GPU, held
 // until the process ends.  Layout: [0, 256) queue counter, [256, 512) the
@@ -405,7 +441,7 @@ ppc::ReactParamsDev to_dev(const pp_chem_config& c) {
   P.rk64 = {c.rk64_atol, c.rk64_nsubsteps_guess, c.rk64_nsubsteps_min, c.rk64_nsubsteps_max};
   P.adapt = {c.rtol, c.atol, c.first_step, c.max_steps, c.batched_lu_batch, c.batched_lu_wait,
              c.batched_waves_per_simd, c.batched_cells_per_wave, c.cvode_max_order, c.max_nls_iters,
-             c.jac_max_age, c.mode == PP_CHEM_CVODE ? 1 : 0, nullptr};
+             c.jac_max_age, (c.mode == PP_CHEM_CVODE || c.mode == PP_CHEM_ARKODE) ? 1 : 0, nullptr};
   return P;
 }
 
@@ -464,6 +500,7 @@ int pp_chem_list_mechanisms(char* buf, int buflen) {
 #define PPC_MECH(M) s += (s.empty() ? "" : ",") + std::string(#M);
 #include "inst/registry.inc"
 #undef PPC_MECH
+  s += ",Manifold";
   if (buf && buflen > 0) {
     std::strncpy(buf, s.c_str(), (size_t)buflen - 1);
     buf[buflen - 1] = '\0';
@@ -473,10 +510,12 @@ int pp_chem_list_mechanisms(char* buf, int buflen) {
 
 int pp_chem_config_init(pp_chem_config* cfg, const char* mechanism, int mode, int reactor_type) {
   if (!cfg) return PP_CHEM_EINVAL;
-  const MechLauncher* L = find_mech(mechanism);
+  const MechLauncher* L = resolve_mech(mechanism, 1);
   if (!L) return PP_CHEM_EMECH;
   std::memset(cfg, 0, sizeof(*cfg));
   cfg->mechanism = L->name;
+  // MANIFOLD_DIM defaults to 1 (Source/Eos/Manifold.H:13-15)
+  if (is_manifold_name(mechanism)) cfg->manifold_dim = 1;
   cfg->reactor_type = reactor_type;
   cfg->mode = mode;
   cfg->mod_reactor = 0;
@@ -511,17 +550,38 @@ int pp_chem_config_init(pp_chem_config* cfg, const char* mechanism, int mode, in
     cfg->max_nls_iters = 3;
     cfg->jac_max_age = 100;
   }
+  if (mode == PP_CHEM_ARKODE) {
+    // ReactorArkode defaults (ReactorArkode.H:132-136): rtol 1e-6, atol 1e-10,
+    // ARKODE_ZONNEVELD_5_3_4 with the PID controller; ARKODE's 500 steps per call
+    cfg->rtol = 1e-6;
+    cfg->atol = 1e-10;
+    cfg->max_steps = 500;
+    cfg->erk_method = 40;
+    cfg->erk_controller = 0;
+  }
   return PP_CHEM_OK;
 }
 
 int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
   if (!cfg || !out) return PP_CHEM_EINVAL;
   *out = nullptr;
-  const MechLauncher* L = find_mech(cfg->mechanism);
+  const bool manifold = is_manifold_name(cfg->mechanism);
+  if (manifold && (cfg->manifold_dim < 1 || cfg->manifold_dim > 5)) return PP_CHEM_EINVAL;
+  const MechLauncher* L = resolve_mech(cfg->mechanism, cfg->manifold_dim);
   if (!L) return PP_CHEM_EMECH;
+  // the Manifold EOS is propagated through ReactorRK64 only (ReactorBase.H:84-97)
+  if (manifold && cfg->mode != PP_CHEM_RK64) return PP_CHEM_EINVAL;
   // ReactorTypes::check_reactor_type (ReactorTypes.H:12-19) as an error code
   if (cfg->reactor_type != 1 && cfg->reactor_type != 2) return PP_CHEM_EINVAL;
-  if (cfg->mode < PP_CHEM_BDF || cfg->mode > PP_CHEM_CVODE) return PP_CHEM_EINVAL;
+  if (cfg->mode < PP_CHEM_BDF || cfg->mode > PP_CHEM_ARKODE) return PP_CHEM_EINVAL;
+  // ReactorArkode's table and controller (ReactorArkode.cpp:26-110); the
+  // ARK437L2SA and ARK548L2SAb tables are not provided
+  if (cfg->mode == PP_CHEM_ARKODE) {
+    if (cfg->erk_method == 43 || cfg->erk_method == 54) return PP_CHEM_EINVAL;
+    if (cfg->erk_controller < 0 || cfg->erk_controller > 3) return PP_CHEM_EINVAL;
+  } else if (cfg->erk_method != 0 || cfg->erk_controller != 0) {
+    return PP_CHEM_EINVAL;
+  }
   // CVODE-style options: the plain FP64 adaptive BDF and ReactorCvode only
   if (cfg->cvode_max_order < 0 || cfg->cvode_max_order > 5 || cfg->max_nls_iters < 0 || cfg->jac_max_age < 0)
     return PP_CHEM_EINVAL;
@@ -569,7 +629,30 @@ int pp_chem_create(const pp_chem_config* cfg, pp_chem** out) {
   h->cfg.mechanism = h->mech_name.c_str();
   h->cfg.block_size = bs;
   h->L = L;
+  h->manifold = manifold;
   h->P = to_dev(*cfg);
+  if (cfg->mode == PP_CHEM_ARKODE) {
+    // unknown codes select ARKODE_ZONNEVELD_5_3_4, as ReactorArkode.cpp:78-81 does
+    const ppc::ErkTableHost* T = nullptr;
+    for (const auto& t : ppc::ERK_TABLES)
+      if (t.code == cfg->erk_method) T = &t;
+    for (const auto& t : ppc::ERK_TABLES)
+      if (!T && t.code == ppc::ERK_DEFAULT) T = &t;
+    const int s = T->s;
+    std::vector<double> tab;
+    tab.insert(tab.end(), T->c, T->c + s);
+    tab.insert(tab.end(), T->A, T->A + s * s);
+    tab.insert(tab.end(), T->b, T->b + s);
+    tab.insert(tab.end(), T->d, T->d + s);
+    hipError_t et = hipMalloc(&h->d_erk, sizeof(double) * tab.size());
+    if (et == hipSuccess) et = hipMemcpy(h->d_erk, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
+    if (et != hipSuccess) {
+      pp_chem_destroy(h);
+      return PP_CHEM_EHIP;
+    }
+    h->P.erk = {h->d_erk, s, T->q, T->p, T->fsal, cfg->erk_controller};
+    h->cfg.erk_method = T->code;
+  }
   hipError_t e = hipGetDevice(&h->device);
   if (e == hipSuccess && cfg->batched_waves_per_cu > 0) {
     int ncu = 0;
@@ -607,6 +690,8 @@ void pp_chem_destroy(pp_chem* h) {
   if (h->d_order) (void)hipFree(h->d_order);
   if (h->d_stage) (void)hipFree(h->d_stage);
   if (h->d_atolv) (void)hipFree(h->d_atolv);
+  if (h->d_man) (void)hipFree(h->d_man);
+  if (h->d_erk) (void)hipFree(h->d_erk);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   delete h;
@@ -617,7 +702,7 @@ int pp_chem_set_typ_vals(pp_chem* h, const double* typ_vals, int n) {
   if (!typ_vals || n != h->L->ns + 1) return fail(h, PP_CHEM_EINVAL, "typ_vals needs NUM_SPECIES + 1 values");
   // read by ReactorCvode only, as the reference's ReactorBDF / ReactorRK64
   // ignore them; typ_vals[0] <= 0 restores the scalar atol
-  if (h->cfg.mode != PP_CHEM_CVODE) return PP_CHEM_OK;
+  if (h->cfg.mode != PP_CHEM_CVODE && h->cfg.mode != PP_CHEM_ARKODE) return PP_CHEM_OK;
   if (!(typ_vals[0] > 0.0)) {
     h->P.adapt.atolv = nullptr;
     drop_graph(h);
@@ -637,6 +722,58 @@ int pp_chem_set_typ_vals(pp_chem* h, const double* typ_vals, int n) {
   return PP_CHEM_OK;
 }
 
+int pp_chem_set_manifold(pp_chem* h, const pp_chem_manifold* m) {
+  if (!h) return PP_CHEM_EINVAL;
+  if (!h->manifold) return fail(h, PP_CHEM_EINVAL, "manifold parameters need the Manifold EOS (mechanism \"Manifold\")");
+  if (!m || !m->dim_lengths || !m->grids || !m->values) return fail(h, PP_CHEM_EINVAL, "null argument");
+  if (m->ndim != h->cfg.manifold_dim)
+    return fail(h, PP_CHEM_EINVAL, "table dimension differs from manifold_dim (NUM_SPECIES - 1)");
+  if (m->nvar < 1) return fail(h, PP_CHEM_EINVAL, "no tabulated variables");
+  ppc::ManifoldDev d{};
+  d.ndim = m->ndim;
+  d.nvar = m->nvar;
+  long long ngrid = 0, stride = 1;
+  for (int k = 0; k < m->ndim; ++k) {
+    const int n = m->dim_lengths[k];
+    if (n < 2) return fail(h, PP_CHEM_EINVAL, "a table dimension has fewer than 2 points");
+    for (int i = 0; i + 1 < n; ++i)
+      if (!(m->grids[ngrid + i + 1] > m->grids[ngrid + i]))
+        return fail(h, PP_CHEM_EINVAL, "table grids must be strictly ascending");
+    d.dim_len[k] = n;
+    d.dim_off[k] = (int)ngrid;
+    d.dim_stride[k] = (int)stride;
+    ngrid += n;
+    stride *= n;
+    if (stride > 0x7fffffffLL / 8) return fail(h, PP_CHEM_EINVAL, "table too large");
+  }
+  d.var_stride = (int)stride;
+  const auto var_ok = [&](int v, bool opt) { return (opt && v == -1) || (v >= 0 && v < m->nvar); };
+  if (!var_ok(m->idx_density, false) || !var_ok(m->idx_T, false))
+    return fail(h, PP_CHEM_EINVAL, "density or temperature variable index out of range");
+  for (int k = 0; k < m->ndim; ++k)
+    if (!var_ok(m->idx_src[k], true)) return fail(h, PP_CHEM_EINVAL, "source variable index out of range");
+  if (m->density_lookup < 0 || m->density_lookup > 2) return fail(h, PP_CHEM_EINVAL, "bad density lookup type");
+  d.idx_density = m->idx_density;
+  d.dens_lookup = m->density_lookup;
+  d.idx_T = m->idx_T;
+  d.compute_T = m->compute_temperature ? 1 : 0;
+  for (int k = 0; k < ppc::MAN_MAXD; ++k) d.idx_src[k] = k < m->ndim ? m->idx_src[k] : -1;
+  d.pnom = m->nominal_pressure;
+  const size_t nval = (size_t)m->nvar * (size_t)stride;
+  HIPCHK(h, hipSetDevice(h->device));
+  if (h->d_man) HIPCHK(h, hipFree(h->d_man));
+  h->d_man = nullptr;
+  h->P.man = ppc::ManifoldDev{};
+  HIPCHK(h, hipMalloc(&h->d_man, sizeof(double) * ((size_t)ngrid + nval)));
+  HIPCHK(h, hipMemcpy(h->d_man, m->grids, sizeof(double) * (size_t)ngrid, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->d_man + ngrid, m->values, sizeof(double) * nval, hipMemcpyHostToDevice));
+  d.grids = h->d_man;
+  d.values = h->d_man + ngrid;
+  h->P.man = d;
+  drop_graph(h);
+  return PP_CHEM_OK;
+}
+
 int pp_chem_num_species(const pp_chem* h) { return h ? h->L->ns : PP_CHEM_EINVAL; }
 int pp_chem_num_reactions(const pp_chem* h) { return h ? h->L->nr : PP_CHEM_EINVAL; }
 long long pp_chem_workspace_rows(const pp_chem* h) {
@@ -649,6 +786,7 @@ long long pp_chem_workspace_rows(const pp_chem* h) {
 int pp_chem_adjoint_1d(pp_chem* h, double* y, const double* rX, int ncells, int nmacro, double dt,
                        int spec_a, int spec_b, double* out, double* grad, void* stream) {
   if (!h) return PP_CHEM_EINVAL;
+  if (int r = manifold_refuses(h, "the adjoint")) return r;
   if (ncells < 0 || nmacro < 1 || !(dt > 0)) return fail(h, PP_CHEM_EINVAL, "bad argument");
   if (ncells > 0 && (!y || !rX || !out || !grad)) return fail(h, PP_CHEM_EINVAL, "null array");
   if (spec_a < 0 || spec_a >= h->L->ns || spec_b < 0 || spec_b >= h->L->ns)
@@ -691,6 +829,7 @@ int pp_chem_react_box(pp_chem* h, const int lo[3], const int hi[3], const pp_arr
                       double* dt_react, double* time, void* stream) {
   (void)mask;  // ignored by the reference (ReactorBDF.cpp:322)
   if (!h) return PP_CHEM_EINVAL;
+  if (int r = manifold_unset(h)) return r;
   if (!lo || !hi || !rY || !rYsrc || !T || !rE || !rEsrc || !dt_react || !time)
     return fail(h, PP_CHEM_EINVAL, "null argument");
   if (!rY->p || !rYsrc->p || !T->p || !rE->p || !rEsrc->p) return fail(h, PP_CHEM_EINVAL, "null array");
@@ -755,6 +894,7 @@ int pp_chem_react_box(pp_chem* h, const int lo[3], const int hi[3], const pp_arr
 int pp_chem_react_1d(pp_chem* h, double* y, double* ysrc, double* rX, double* rXsrc, double* dt_react,
                      double* time, int ncells, int on_device, void* stream) {
   if (!h) return PP_CHEM_EINVAL;
+  if (int r = manifold_unset(h)) return r;
   if (!dt_react || !time || ncells < 0) return fail(h, PP_CHEM_EINVAL, "bad argument");
   if (ncells > 0 && (!y || !ysrc || !rX || !rXsrc)) return fail(h, PP_CHEM_EINVAL, "null array");
   const int ns = h->L->ns, n = ns + 1;
@@ -801,6 +941,7 @@ int pp_chem_react_1d(pp_chem* h, double* y, double* ysrc, double* rX, double* rX
 int pp_chem_react_1d_noalloc(pp_chem* h, double* y, double* ysrc, double* rX, double* rXsrc, double* dt_react,
                              double* time, int ncells, void* stream) {
   if (!h) return PP_CHEM_EINVAL;
+  if (int r = manifold_unset(h)) return r;
   if (!dt_react || !time || ncells < 0) return fail(h, PP_CHEM_EINVAL, "bad argument");
   if (ncells > 0 && (!y || !ysrc || !rX || !rXsrc)) return fail(h, PP_CHEM_EINVAL, "null array");
   if (!h->scratch) return fail(h, PP_CHEM_EHIP, "device scratch of pp_chem_react_1d_noalloc unavailable");
@@ -862,6 +1003,7 @@ int pp_chem_flatten(pp_chem* h, const int lo[3], const int hi[3], const pp_array
                     const pp_array4* T, const pp_array4* rE, const pp_array4* rEsrc, double* y_vect,
                     double* src_vect, double* vect_energy, double* src_vect_energy, void* stream) {
   if (!h) return PP_CHEM_EINVAL;
+  if (int r = manifold_refuses(h, "flatten")) return r;
   if (!lo || !hi || !rY || !rYsrc || !T || !rE || !rEsrc) return fail(h, PP_CHEM_EINVAL, "null argument");
   if (!rY->p || !rYsrc->p || !T->p || !rE->p || !rEsrc->p) return fail(h, PP_CHEM_EINVAL, "null array");
   ppc::FlatArgs a{};
@@ -886,6 +1028,7 @@ int pp_chem_unflatten(pp_chem* h, const int lo[3], const int hi[3], const pp_arr
                       const pp_array4* rE, const pp_array4* rEsrc, const pp_array4* FC, const double* y_vect,
                       const double* vect_energy, const long long* FCunt, double dt, void* stream) {
   if (!h) return PP_CHEM_EINVAL;
+  if (int r = manifold_refuses(h, "unflatten")) return r;
   if (!lo || !hi || !rY || !T || !rE || !rEsrc || !FC) return fail(h, PP_CHEM_EINVAL, "null argument");
   if (!rY->p || !T->p || !rE->p || !rEsrc->p || !FC->p) return fail(h, PP_CHEM_EINVAL, "null array");
   ppc::FlatArgs a{};
@@ -908,6 +1051,7 @@ int pp_chem_unflatten(pp_chem* h, const int lo[3], const int hi[3], const pp_arr
 int pp_chem_react_1d_chained(pp_chem* h, double* y, double* ysrc, double* rX, double* rXsrc, double* dt_react,
                              double* time, int ncells, int ncalls, void* stream) {
   if (!h) return PP_CHEM_EINVAL;
+  if (int r = manifold_unset(h)) return r;
   if (!dt_react || !time || ncells < 0 || ncalls < 1) return fail(h, PP_CHEM_EINVAL, "bad argument");
   if (ncells > 0 && (!y || !ysrc || !rX || !rXsrc)) return fail(h, PP_CHEM_EINVAL, "null array");
   hipStream_t s = (hipStream_t)stream;
@@ -972,6 +1116,7 @@ int pp_chem_react_1d_chained(pp_chem* h, double* y, double* ysrc, double* rX, do
 
 int pp_chem_production_rate(pp_chem* h, const double* sc, const double* T, int ncells, double* wdot, void* stream) {
   if (!h) return PP_CHEM_EINVAL;
+  if (int r = manifold_refuses(h, "productionRate")) return r;
   if (ncells < 0 || (ncells > 0 && (!sc || !T || !wdot))) return fail(h, PP_CHEM_EINVAL, "bad argument");
   hipStream_t s = (hipStream_t)stream;
   const hipError_t e = h->L->mech_eval(0, sc, T, ncells, wdot, s);
@@ -982,6 +1127,7 @@ int pp_chem_production_rate(pp_chem* h, const double* sc, const double* T, int n
 
 int pp_chem_progress_rate(pp_chem* h, const double* sc, const double* T, int ncells, double* q, void* stream) {
   if (!h) return PP_CHEM_EINVAL;
+  if (int r = manifold_refuses(h, "the rates of progress")) return r;
   if (ncells < 0 || (ncells > 0 && (!sc || !T || !q))) return fail(h, PP_CHEM_EINVAL, "bad argument");
   hipStream_t s = (hipStream_t)stream;
   const hipError_t e = h->L->mech_eval(3, sc, T, ncells, q, s);
@@ -993,6 +1139,7 @@ int pp_chem_progress_rate(pp_chem* h, const double* sc, const double* T, int nce
 int pp_chem_ajacobian(pp_chem* h, const double* sc, const double* T, int ncells, int consP, double* J,
                       void* stream) {
   if (!h) return PP_CHEM_EINVAL;
+  if (int r = manifold_refuses(h, "aJacobian")) return r;
   if (ncells < 0 || (consP != 0 && consP != 1) || (ncells > 0 && (!sc || !T || !J)))
     return fail(h, PP_CHEM_EINVAL, "bad argument");
   hipStream_t s = (hipStream_t)stream;

@@ -12,6 +12,7 @@
 // array of m floats takes ceil(m / 2) rows.
 #pragma once
 #include <type_traits>
+#include "erk_tables.h"
 #include "ppc_common.h"
 #include "reactors_fwd.h"
 
@@ -65,6 +66,7 @@ PPC_DI auto lane_array(T* priv, const LaneWS& w, long long row) {
 template <class M>
 constexpr long long integrator_ws_rows(int mode) {
   constexpr long long n = M::NEQ, nn = n * n;
+  if (mode == MODE_ERK) return ERK_MAX_STAGES * n;  // the stages, every mechanism
   if (!uses_ws<M>()) return 0;
   switch (mode) {
     case MODE_BDF: return nn + 2 * (n + 1) * n;                   // A, Krylov basis V, Hessenberg H

@@ -84,6 +84,7 @@ hipError_t launch_1d(int mode, const ReactParamsDev& P, double* y, const double*
     case MODE_BDF: return launch_1d_mode<M, MODE_BDF>(P, y, ysrc, rX, rXsrc, dt, ncells, st, block, s);
     case MODE_RK64: return launch_1d_mode<M, MODE_RK64>(P, y, ysrc, rX, rXsrc, dt, ncells, st, block, s);
     case MODE_ADAPTIVE: return launch_1d_mode<M, MODE_ADAPTIVE>(P, y, ysrc, rX, rXsrc, dt, ncells, st, block, s);
+    case MODE_ERK: return launch_1d_mode<M, MODE_ERK>(P, y, ysrc, rX, rXsrc, dt, ncells, st, block, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -95,6 +96,7 @@ hipError_t launch_box(int mode, const ReactParamsDev& P, const BoxArgs& b, doubl
     case MODE_BDF: return launch_box_mode<M, MODE_BDF>(P, b, dt, ncells, st, block, s);
     case MODE_RK64: return launch_box_mode<M, MODE_RK64>(P, b, dt, ncells, st, block, s);
     case MODE_ADAPTIVE: return launch_box_mode<M, MODE_ADAPTIVE>(P, b, dt, ncells, st, block, s);
+    case MODE_ERK: return launch_box_mode<M, MODE_ERK>(P, b, dt, ncells, st, block, s);
     default: return hipErrorInvalidValue;
   }
 }

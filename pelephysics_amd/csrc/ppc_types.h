@@ -27,15 +27,46 @@ struct AdaptParamsDev {  // adaptive BDF tolerances
   const double* atolv;    // per-component absolute tolerances atol x typical value (device, NEQ)
 };
 
+// A tabulated manifold model (TabulatedFunctionData, Source/Utility/
+// BlackBoxFunction/Table.H:11-17, with the Manifold EOS parameters of
+// Source/Eos/EosParams.H:39-49): ndim table dimensions (the manifold
+// parameters), nvar tabulated variables on the tensor grid, first dimension
+// fastest.
+constexpr int MAN_MAXD = 5;  // MAXD_TABLE (Table.H:4)
+struct ManifoldDev {
+  int ndim, nvar;
+  int dim_len[MAN_MAXD];     // grid points per dimension (>= 2)
+  int dim_off[MAN_MAXD];     // offset of each dimension's grid in grids
+  int dim_stride[MAN_MAXD];  // values between neighbours along each dimension
+  int var_stride;            // values per variable
+  const double* grids;       // device, sum of dim_len
+  const double* values;      // device, nvar x var_stride
+  int idx_density;           // variable holding RHO, lnRHO or invRHO
+  int dens_lookup;           // 0 linear, 1 log, 2 inverse (density_lookup_type)
+  int idx_T;                 // variable holding T
+  int compute_T;             // 1: T from the table; 0: T = 1
+  int idx_src[MAN_MAXD];     // source term of each manifold parameter, -1: none (zero)
+  double pnom;               // nominal pressure, dyn/cm^2
+};
+
+struct ErkParamsDev {  // ReactorArkode: the table of ode.rk_method and ode.rk_controller
+  const double* tab;   // device: c[s], A[s*s] (row-major), b[s], d[s]
+  int s, q, p, fsal;   // stages, order, embedding order, first same as last
+  int controller;      // 0 PID, 1 PI, 2 I, 3 explicit Gustafsson
+};
+
 struct ReactParamsDev {
   int reactor_type;
   BDFParamsDev bdf;
   RK64ParamsDev rk64;
-  AdaptParamsDev adapt;
+  AdaptParamsDev adapt;  // tolerances, step limit and typical values also of ReactorArkode
+  ErkParamsDev erk;
   // per-lane matrix workspace of the larger mechanisms (lane_ws.h): ws_lanes
   // lanes x the integrator's rows, or null
   double* ws;
   long long ws_lanes;
+  // the manifold model of the Manifold EOS (null values: none set)
+  ManifoldDev man;
 };
 
 }  // namespace ppc

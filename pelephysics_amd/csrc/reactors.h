@@ -10,6 +10,7 @@
 //    Newton with a dense FP64 LU and Jacobian reuse.
 #pragma once
 #include "chem_cell.h"
+#include "erk.h"
 #include "lane_ws.h"
 #include "reactors_fwd.h"
 
@@ -234,11 +235,10 @@ PPC_DI double rk_err(int s) {
   return e[s];
 }
 
-template <class M>
-PPC_DI int rk64_cell(const RK64ParamsDev& p, const int reactor_type, double* __restrict__ y,
-                     const double rhoe_init, const double rhoesrc, const double* __restrict__ rYsrc,
-                     const double dt_react) {
-  constexpr int n = M::NEQ;
+// The stepping of ReactorRK64::react on n equations; rhs(t, y, ydot) is the
+// right-hand side at time t from the start of the call.
+template <int n, class RHS>
+PPC_DI int rk64_core(const RK64ParamsDev& p, double* __restrict__ y, const double dt_react, RHS&& rhs) {
   double carry[n], err[n], yd[n];
   const double time_out = dt_react;
   double current_time = 0.0;
@@ -250,7 +250,7 @@ PPC_DI int rk64_cell(const RK64ParamsDev& p, const int reactor_type, double* __r
   while (current_time < time_out) {
     for (int i = 0; i < n; ++i) err[i] = 0.0;
     for (int st = 0; st < 6; ++st) {
-      f_kernel_spec<M>(current_time, reactor_type, y, yd, rhoe_init, rhoesrc, rYsrc);
+      rhs(current_time, y, yd);
       const double a = rk_alpha(st), b = rk_beta(st), e = rk_err(st);
       for (int i = 0; i < n; ++i) {
         err[i] += e * dt_rk * yd[i];
@@ -270,6 +270,15 @@ PPC_DI int rk64_cell(const RK64ParamsDev& p, const int reactor_type, double* __r
     dt_rk = fmin(dt_rk, time_out - current_time);
   }
   return nsteps;
+}
+
+template <class M>
+PPC_DI int rk64_cell(const RK64ParamsDev& p, const int reactor_type, double* __restrict__ y,
+                     const double rhoe_init, const double rhoesrc, const double* __restrict__ rYsrc,
+                     const double dt_react) {
+  return rk64_core<M::NEQ>(p, y, dt_react, [&](double t, const double* yy, double* yd) {
+    f_kernel_spec<M>(t, reactor_type, yy, yd, rhoe_init, rhoesrc, rYsrc);
+  });
 }
 
 // ------------------------------------------------ adaptive BDF (FP64) ---
@@ -668,9 +677,13 @@ PPC_DI int adaptive_cell(const AdaptParamsDev& p, const int reactor_type, double
 template <class M, int MODE>
 PPC_DI int integrate_cell(const ReactParamsDev& P, double* y, double rhoe, double rhoesrc,
                           const double* src, double dt, int* cnt, const LaneWS& w) {
-  if (MODE == MODE_BDF) return bdf_parity_cell<M>(P.bdf, P.reactor_type, y, rhoe, rhoesrc, src, dt, w);
-  if (MODE == MODE_RK64) return rk64_cell<M>(P.rk64, P.reactor_type, y, rhoe, rhoesrc, src, dt);
-  return adaptive_cell<M, double>(P.adapt, P.reactor_type, y, rhoe, rhoesrc, src, dt, cnt, w);
+  if constexpr (MODE == MODE_ERK) {
+    return erk_cell<M>(P, P.reactor_type, y, rhoe, rhoesrc, src, dt, cnt, w);
+  } else {
+    if (MODE == MODE_BDF) return bdf_parity_cell<M>(P.bdf, P.reactor_type, y, rhoe, rhoesrc, src, dt, w);
+    if (MODE == MODE_RK64) return rk64_cell<M>(P.rk64, P.reactor_type, y, rhoe, rhoesrc, src, dt);
+    return adaptive_cell<M, double>(P.adapt, P.reactor_type, y, rhoe, rhoesrc, src, dt, cnt, w);
+  }
 }
 
 PPC_DI void store_counters(const CellStats& st, int c, int nsteps, const int* cnt) {

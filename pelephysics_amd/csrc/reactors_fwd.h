@@ -7,8 +7,9 @@ namespace ppc {
 
 // integrators: 0 ReactorBDF parity, 1 ReactorRK64 parity, 2 adaptive BDF (plain,
 // one lane per cell, FP64), 3 batched adaptive BDF with FP32 LU, 4 batched
-// adaptive BDF with FP64 LU
-enum : int { MODE_BDF = 0, MODE_RK64 = 1, MODE_ADAPTIVE = 2, MODE_BATCHED_MIXED = 3, MODE_BATCHED_FP64 = 4 };
+// adaptive BDF with FP64 LU, 6 ReactorArkode's explicit Runge-Kutta
+enum : int { MODE_BDF = 0, MODE_RK64 = 1, MODE_ADAPTIVE = 2, MODE_BATCHED_MIXED = 3, MODE_BATCHED_FP64 = 4,
+             MODE_ERK = 6 };
 
 // ------------------------------------------------------------------ I/O --
 struct A4 {  // amrex::Array4 layout

@@ -4,7 +4,8 @@ Reference interface (Source/Reactions/ReactorBase.H:14-106, Source/Factory.H):
 
 * ``ReactorBase.create(name)`` selects an integrator by its identifier
   ("ReactorBDF", "ReactorRK64", "ReactorCvode": ReactorCvode's options on
-  this project's FP64 adaptive BDF) plus this project's adaptive forms
+  this project's FP64 adaptive BDF, "ReactorArkode": its explicit RK tables
+  and step controllers) plus this project's adaptive forms
   ("ReactorBDFAdaptive", "ReactorBDFAdaptiveMixed", "ReactorBDFBatchedFP64");
   unknown names raise with
   the list of valid names (``Factory.H:135-147`` aborts with the same list);
@@ -33,7 +34,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # the in-tree build; PELE_CHEM_HIP_LIB selects another build of the same library
 LIB_PATH = os.environ.get("PELE_CHEM_HIP_LIB") or os.path.join(HERE, "libpele_chem_hip.so")
 
-BDF, RK64, BDF_ADAPTIVE, BDF_ADAPTIVE_MIXED, BDF_BATCHED_FP64, CVODE = 0, 1, 2, 3, 4, 5
+BDF, RK64, BDF_ADAPTIVE, BDF_ADAPTIVE_MIXED, BDF_BATCHED_FP64, CVODE, ARKODE = 0, 1, 2, 3, 4, 5, 6
 E_REACTOR, H_REACTOR = 1, 2  # Source/Reactions/ReactorTypes.H:10-11
 
 IDENTIFIERS = {
@@ -43,6 +44,7 @@ IDENTIFIERS = {
     "ReactorBDFAdaptiveMixed": BDF_ADAPTIVE_MIXED,
     "ReactorBDFBatchedFP64": BDF_BATCHED_FP64,
     "ReactorCvode": CVODE,
+    "ReactorArkode": ARKODE,
 }
 
 ERRORS = {-1: "EINVAL", -2: "EMECH", -3: "EHIP", -4: "EINTEGRATE"}
@@ -70,6 +72,7 @@ class Config(C.Structure):
         ("batched_lu_batch", C.c_int), ("batched_lu_wait", C.c_int), ("batched_box_direct", C.c_int),
         ("batched_waves_per_simd", C.c_int), ("batched_cells_per_wave", C.c_int),
         ("cvode_max_order", C.c_int), ("max_nls_iters", C.c_int), ("jac_max_age", C.c_int),
+        ("manifold_dim", C.c_int), ("erk_method", C.c_int), ("erk_controller", C.c_int),
     ]
 
 
@@ -108,6 +111,11 @@ _KEYS = {
     # cvode.max_substeps, ode.max_nls_iters, the Jacobian's largest age
     "max_order": "cvode_max_order", "max_substeps": "max_steps", "max_nls_iters": "max_nls_iters",
     "jac_max_age": "jac_max_age",
+    # mechanism "Manifold": MANIFOLD_DIM, the number of manifold parameters
+    "manifold_dim": "manifold_dim",
+    # ReactorArkode (ReactorArkode.cpp:12-18): ode.rk_method, ode.rk_controller;
+    # ode.use_erkstep selects ERKStep or ARKStep there, both explicit here
+    "rk_method": "erk_method", "rk_controller": "erk_controller", "use_erkstep": None,
 }
 
 _lib = None
@@ -160,6 +168,8 @@ def library() -> C.CDLL:
         L.pp_chem_progress_rate.argtypes = [vp, vp, vp, C.c_int, vp, vp]
         L.pp_chem_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.pp_chem_copy_cell_steps.argtypes = [vp, vp, C.c_int]
+        from .manifold import PPManifold
+        L.pp_chem_set_manifold.argtypes = [vp, C.POINTER(PPManifold)]
         L.pp_chem_last_error.argtypes = [vp]
         L.pp_chem_last_error.restype = C.c_char_p
         _lib = L
@@ -254,6 +264,8 @@ class ReactorBase:
             if k == "atol":
                 cfg.rk64_atol = float(v)
                 cfg.atol = float(v)
+            elif k == "use_erkstep":
+                continue
             else:
                 setattr(cfg, _KEYS[k], v)
         self._cfg = cfg
@@ -289,6 +301,14 @@ class ReactorBase:
         if getattr(eosparm, "identifier", None) == "Manifold":
             raise ReactorError(-1, "Manifold EOS only supported with ReactorRK64 and ReactorNull for now")
         self.eosparm = eosparm
+
+    def _set_manifold(self, parm) -> None:
+        m, keep = parm.c_struct()
+        rc = library().pp_chem_set_manifold(self._h, C.byref(m))
+        del keep
+        if rc != 0:
+            raise ReactorError(rc, library().pp_chem_last_error(self._h).decode())
+        self.eosparm = parm
 
     def close(self):
         if self._h:
@@ -531,6 +551,18 @@ class ReactorRK64(ReactorBase):
     identifier = "ReactorRK64"
     _mode = RK64
 
+    def set_eos_parm(self, eosparm):
+        """ReactorRK64::set_eos_parm (ReactorRK64.H:123-128) keeps the EOS
+        parameters for its RHS: with mechanism "Manifold" a
+        pelephysics_amd.manifold.ManifoldParm (the table every lookup of the
+        next calls interpolates in); the ideal-gas EOS takes none."""
+        if getattr(eosparm, "identifier", None) == "Manifold":
+            if self.mechanism != "Manifold":
+                raise ReactorError(-1, "the Manifold EOS needs mechanism \"Manifold\" (Chemistry_Model=Null)")
+            self._set_manifold(eosparm)
+        else:
+            self.eosparm = eosparm
+
 
 class ReactorBDFAdaptive(ReactorBase):
     identifier = "ReactorBDFAdaptive"
@@ -559,5 +591,15 @@ class ReactorCvode(ReactorBase):
     _mode = CVODE
 
 
+class ReactorArkode(ReactorBase):
+    """ReactorArkode's explicit Runge-Kutta tables (ode.rk_method, default 40:
+    ARKODE_ZONNEVELD_5_3_4) and step controllers (ode.rk_controller, default
+    0: PID) with rtol 1e-6 / atol 1e-10 (x typical values), each cell with its
+    own step size; FC and the return value count RHS evaluations."""
+    identifier = "ReactorArkode"
+    _mode = ARKODE
+
+
 _REGISTRY = {c.identifier: c for c in (ReactorBDF, ReactorRK64, ReactorBDFAdaptive,
-                                       ReactorBDFAdaptiveMixed, ReactorBDFBatchedFP64, ReactorCvode)}
+                                       ReactorBDFAdaptiveMixed, ReactorBDFBatchedFP64, ReactorCvode,
+                                       ReactorArkode)}

@@ -7,6 +7,7 @@
 // against the reference signatures and exercise registration and error
 // handling; it is not an AMReX or PelePhysics build.
 #pragma once
+#include <cstring>
 #include <functional>
 #include <map>
 #include <memory>
@@ -63,6 +64,10 @@ struct GpuArray {
   T& operator[](int i) { return arr[i]; }
   const T& operator[](int i) const { return arr[i]; }
 };
+namespace Gpu {
+// AMReX_GpuDevice.H: device -> host copy (a plain copy on a CPU build)
+inline void dtoh_memcpy(void* dst, const void* src, std::size_t n) { std::memcpy(dst, src, n); }
+}  // namespace Gpu
 }  // namespace amrex
 
 // the species count of the compiled mechanism (mechanism.H); LiDryer here
@@ -76,6 +81,53 @@ struct GpuArray {
   } while (0)
 
 namespace pele::physics {
+
+// The EOS parameter types the Manifold path of the adapter reads
+// (Source/Utility/BlackBoxFunction/BlackBoxFunction.H:9-24, Table.H:11-17,
+// Source/Eos/EosParams.H:15-50) and the build's EOS choice (PhysicsType).
+enum class BlackBoxModel { TABLE = 0, NEURAL_NET = 1 };
+struct BlackBoxFunctionData {
+  BlackBoxModel bbmodel;
+  static constexpr int len_str = 64;
+  char model_name[len_str + 1];
+  int Ndim;
+  int Nvar;
+  int Nmanpar;
+  char* dimnames;
+  char* varnames;
+};
+struct TabulatedFunctionData : BlackBoxFunctionData {
+  int varSpacing;
+  int* dimLengths;
+  int* dimDataSpacing;
+  double* grids;
+  double* values;
+};
+namespace eos {
+enum class density_lookup_type { linear, log, inverse };
+struct Fuego {};
+struct Manifold {};
+template <class EOS>
+struct EosParm {};
+template <>
+struct EosParm<Manifold> {
+  const BlackBoxFunctionData* manf_data;
+  double Pnom_cgs;
+  density_lookup_type dens_lookup;
+  bool compute_temperature{false};
+  int idx_density{0};
+  int idx_T{0};
+  int idx_Wdot[NUM_SPECIES - 1];
+  bool has_mani_src{true};
+};
+}  // namespace eos
+struct PhysicsType {
+#ifdef USE_MANIFOLD_EOS
+  using eos_type = eos::Manifold;
+#else
+  using eos_type = eos::Fuego;
+#endif
+};
 
 template <class Base, class... Args>
 struct Factory {
@@ -155,9 +207,19 @@ class ReactorBase : public Factory<ReactorBase> {
     for (int i = 0; i < NUM_SPECIES + 1; ++i) m_typ_vals[i] = v[i];
   }
 
+  // ReactorBase.H:86-97
+  virtual void set_eos_parm(const eos::EosParm<PhysicsType::eos_type>* eosparm)
+  {
+    if (std::is_same_v<PhysicsType::eos_type, eos::Manifold>) {
+      amrex::Abort("Manifold EOS only supported with ReactorRK64 and ReactorNull for now");
+    }
+    m_eosparm = eosparm;
+  }
+
  protected:
   int verbose{0};
   amrex::GpuArray<amrex::Real, NUM_SPECIES + 1> m_typ_vals = {{0.0}};
+  const eos::EosParm<PhysicsType::eos_type>* m_eosparm{nullptr};
 };
 
 }  // namespace reactions

@@ -18,6 +18,28 @@ int main() {
   } catch (const std::exception& e) {
     std::printf("abort reactor_type: %s\n", e.what());
   }
+#ifdef USE_MANIFOLD_EOS
+  // a Manifold build: mechanism "Manifold" with NUM_SPECIES - 1 parameters;
+  // ReactorBDF refuses it at init (the library) and at set_eos_parm (the
+  // reference's message); ReactorRK64 reaches the device
+  try {
+    r->init(2, 1);
+  } catch (const std::exception& e) {
+    std::printf("abort manifold bdf init: %s\n", e.what());
+  }
+  pele::physics::eos::EosParm<pele::physics::eos::Manifold> ep{};
+  try {
+    r->set_eos_parm(&ep);
+  } catch (const std::exception& e) {
+    std::printf("abort manifold bdf eos: %s\n", e.what());
+  }
+  try {
+    ReactorBase::create("ReactorRK64_HIP")->init(2, 1);
+    std::printf("manifold rk64 init ok\n");
+  } catch (const std::exception& e) {
+    std::printf("abort manifold rk64 init: %s\n", e.what());
+  }
+#endif
   amrex::ParmParse::table()["ode.mechanism"] = "nope";
   try {
     r->init(2, 1);

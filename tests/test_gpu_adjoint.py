@@ -144,8 +144,9 @@ def test_adjoint_per_parameter_64_cells(models, oracles):
       quotients of outputs near 1e-12, such as Y_CO in cold cells, carry
       errors of a few 1e-4 from the integration's rounding).
 
-    At both steps every resolved g_j is within 1e-3 |fd_j| and 99% of them
-    within 1e-4 |fd_j|."""
+    At both steps every resolved g_j is within 1e-3 |fd_j| + 1e-5 max_j |fd_j|
+    (the adjoint's error is normwise, ~1e-6 of the largest sensitivity of the
+    cell and output) and 99% of them within 1e-4 |fd_j|."""
     torch = _torch()
     from pelephysics_amd.reactor import ReactorBase
     m = models["drm19"]
@@ -187,10 +188,14 @@ def test_adjoint_per_parameter_64_cells(models, oracles):
         print(f"delta {tag}: resolved {int(resolved.sum())} of {resolved.size} sensitivities, "
               f"{within.mean():.4%} within 1e-4, worst {float(rel.max()):.3g}; "
               f"worst normwise {float(normwise[np.abs(fd).max(1, keepdims=True).repeat(nr, 1) > 0].max()):.3g}")
-        # every resolved sensitivity within 1e-3, 99% of them within 1e-4: the
-        # adjoint's error is ~1e-6 of the largest sensitivity of its cell and
-        # output, which is up to 1e-3 of the sensitivities 1e-3 below it
-        assert (rel <= 1e-3).all(), (tag, np.argwhere(rel > 1e-3)[:10])
+        # every resolved sensitivity within 1e-3 |fd_j| + 1e-5 of the largest
+        # sensitivity of its cell and output, 99% of them within 1e-4 |fd_j|:
+        # the adjoint's error is ~1e-6 of that largest sensitivity (normwise),
+        # which is up to 1e-3 of the sensitivities 1e-3 below it and more for
+        # the smaller ones that an FD quotient happens to resolve
+        floor = 1e-5 * np.abs(fd).max(1, keepdims=True)
+        bad = resolved & (err > 1e-3 * np.abs(fd) + floor)
+        assert not bad.any(), (tag, np.argwhere(bad)[:10])
         assert within.mean() >= 0.99, tag
         if tag == "1e-4":
             # Y_CO and Y_OH: nearly every sensitivity above 1e-3 of the cell's

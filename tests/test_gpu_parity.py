@@ -424,7 +424,7 @@ def test_edge_cases(models):
     with pytest.raises(ReactorError):
         ReactorBase.create("ReactorBDF").init(3, 1, mechanism="LiDryer")
     with pytest.raises(ReactorError):
-        ReactorBase.create("ReactorCvode")
+        ReactorBase.create("ReactorNull")
     with pytest.raises(ReactorError):
         ReactorBase.create("ReactorBDF").init(2, 1, mechanism="nope")
     # MOD_REACTOR time semantics (ReactorBDF.cpp:481-483)

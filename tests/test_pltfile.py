@@ -5,7 +5,15 @@ The reference holds no plotfile, so the reader is checked against files of
 the published AMReX format written here: multi-box round trips (bitwise),
 both byte orders, uncovered cells, the MKS -> CGS conversion and component
 ranges of the ReactEval driver, and its abort on a missing variable.  Against
-real PeleLMeX output the reader is unpinned."""
+real PeleLMeX output the reader is unpinned.
+
+fill_patch (PltFileManager::fillPatchFromPlt, PltFileManager.cpp:188-352)
+uses AMReX's interpolaters, which the reference tree does not hold
+(Submodules/amrex is empty): cell-conservative linear interpolation with
+monotonized-central slopes is pinned by its closed forms (linear data
+reproduced, coarse averages conserved, no new extrema, constants kept) and
+the level logic by multi-level files; cell-by-cell parity with AMReX's
+CellConservativeLinear at box and domain edges is unpinned."""
 import os
 
 import numpy as np
@@ -81,3 +89,91 @@ def test_reactor_state_missing_variable(tmp_path, models):
     pltfile.write(p, names, np.ones((len(names), 2, 2, 2)))
     with pytest.raises(pltfile.PltFileError, match="must be in the pltfile"):
         pltfile.reactor_state(pltfile.PlotFile(p), sp)
+
+
+# ---- fill_patch: PltFileManager::fillPatchFromPlt (PltFileManager.cpp:188-352)
+
+def _two_level(tmp_path, coarse, fine_val, boxes, r=2, nv=1):
+    nv, nz, ny, nx = coarse.shape
+    fine = np.full((nv, nz * r, ny * r, nx * r), np.nan)
+    for lo, hi in boxes:
+        fine[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = fine_val
+    p = str(tmp_path / "plt2")
+    pltfile.write(p, [f"v{i}" for i in range(nv)], coarse, max_grid_size=4, fine=[(r, boxes, fine)])
+    return pltfile.PlotFile(p)
+
+
+def test_multi_level_write_read(tmp_path):
+    c = _field(2, 4, 4, 8)
+    P = _two_level(tmp_path, c, 5.0, [((4, 0, 0), (11, 7, 7))])
+    assert P.nlevels == 2 and P.ref_ratio == [2] and P.domain_shape(1) == (16, 8, 8)
+    np.testing.assert_array_equal(P.level(0), c)
+    l1 = P.level(1)
+    assert np.all(l1[:, :, :, 4:12] == 5.0) and np.isnan(l1[:, :, :, :4]).all() and np.isnan(l1[:, :, :, 12:]).all()
+
+
+def _linear(nx, ny, nz, a=(1.0, 3.0, -2.0, 0.5)):
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    z = (np.arange(nz) + 0.5) / nz
+    return (a[0] + a[1] * x[None, None, :] + a[2] * y[None, :, None] + a[3] * z[:, None, None])[None]
+
+
+def test_fill_patch_conservative_linear_closed_forms(tmp_path):
+    """Linear data: the interior fine cells take the field's values at their
+    centres (the limiter leaves linear slopes alone); next to a non-periodic
+    boundary the slope across it is zero; every coarse cell is the
+    average of its fine cells; constants stay constant."""
+    c = _linear(8, 8, 4)
+    p = str(tmp_path / "lin")
+    pltfile.write(p, ["f"], c, max_grid_size=4)
+    P = pltfile.PlotFile(p)
+    f = pltfile.fill_patch(P, (32, 16, 8))
+    exact = _linear(32, 16, 8)
+    inner = (slice(None), slice(2, -2), slice(2, -2), slice(4, -4))
+    np.testing.assert_allclose(f[inner], exact[inner], rtol=0, atol=1e-13)
+    avg = f.reshape(1, 4, 2, 8, 2, 8, 4).mean(axis=(2, 4, 6))
+    np.testing.assert_allclose(avg, c, rtol=0, atol=1e-13)
+    # the first coarse cell in x (fine cells 0-3): no slope in x at the boundary
+    assert np.all(f[0, :, :, 0:4] == f[0, :, :, 0:1]) and np.all(f[0, :, :, -4:] == f[0, :, :, -1:])
+    p2 = str(tmp_path / "const")
+    pltfile.write(p2, ["f"], np.full((1, 4, 8, 8), 3.25), max_grid_size=4)
+    assert np.all(pltfile.fill_patch(pltfile.PlotFile(p2), (16, 16, 8)) == 3.25)
+
+
+def test_fill_patch_is_conservative_and_monotone(tmp_path):
+    """Random data: fine cells average to their coarse cell and stay within
+    the coarse cell's neighbourhood extrema (the limited slopes create no
+    new extrema); piecewise constant (interp_type 0) repeats the parents."""
+    c = _field(1, 4, 6, 8, seed=3)
+    p = str(tmp_path / "rnd")
+    pltfile.write(p, ["f"], c, max_grid_size=4)
+    P = pltfile.PlotFile(p)
+    f = pltfile.fill_patch(P, (16, 12, 8))
+    np.testing.assert_allclose(f.reshape(1, 4, 2, 6, 2, 8, 2).mean(axis=(2, 4, 6)), c, rtol=0, atol=1e-12)
+    from scipy.ndimage import maximum_filter, minimum_filter
+    hi = maximum_filter(c[0], size=3, mode="nearest").repeat(2, 0).repeat(2, 1).repeat(2, 2)
+    lo = minimum_filter(c[0], size=3, mode="nearest").repeat(2, 0).repeat(2, 1).repeat(2, 2)
+    assert np.all(f[0] <= hi + 1e-12) and np.all(f[0] >= lo - 1e-12)
+    pc = pltfile.fill_patch(P, (16, 12, 8), interp_type=0)
+    np.testing.assert_array_equal(pc[0], c[0].repeat(2, 0).repeat(2, 1).repeat(2, 2))
+    per = pltfile.fill_patch(P, (16, 12, 8), periodic=(True, True, True))
+    np.testing.assert_allclose(per.reshape(1, 4, 2, 6, 2, 8, 2).mean(axis=(2, 4, 6)), c, rtol=0, atol=1e-12)
+    assert not np.allclose(per, f)  # boundary slopes differ
+
+
+def test_fill_patch_takes_finer_levels(tmp_path):
+    """A target at the file's level-1 resolution copies level 1 where it
+    covers and interpolates level 0 elsewhere; a finer target interpolates
+    both; a target coarser than level 0 raises as the reference aborts."""
+    c = _linear(8, 4, 4)
+    P = _two_level(tmp_path, c, 7.0, [((4, 0, 0), (9, 7, 7))])
+    f1 = pltfile.fill_patch(P, (16, 8, 8))
+    assert np.all(f1[0, :, :, 4:10] == 7.0)
+    f0 = pltfile.fill_patch(P, (8, 4, 4))  # level 0's own resolution: level 1 is finer, not read
+    np.testing.assert_array_equal(f0, c)
+    f2 = pltfile.fill_patch(P, (32, 16, 16))
+    assert np.all(f2[0, :, :, 8:20] == 7.0) and np.isfinite(f2).all()
+    np.testing.assert_allclose(f2[0, 4:-4, 4:-4, 24:28], _linear(32, 16, 16)[0, 4:-4, 4:-4, 24:28], atol=1e-13)
+    with pytest.raises(pltfile.PltFileError, match="coarser"):
+        pltfile.fill_patch(P, (4, 2, 2))
