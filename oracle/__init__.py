@@ -125,6 +125,8 @@ def lib(variant: str = "fma"):
         for nm in ("ppo_T_given_hy", "ppo_T_given_ey"):
             getattr(L, nm).argtypes = [vp, C.c_double, _dp, C.c_double, C.POINTER(C.c_int)]
             getattr(L, nm).restype = C.c_double
+        L.ppo_tinv_count.argtypes = [C.c_int]
+        L.ppo_tinv_stats.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.ppo_production_rate.argtypes = [vp, _dp, C.c_double, _dp]
         L.ppo_progress_rates.argtypes = [vp, _dp, C.c_double, _dp]
         L.ppo_jacobian.argtypes = [vp, _dp, C.c_double, C.c_int, _dp]
@@ -437,6 +439,31 @@ class Mech:
                                 rY.reshape(-1), rYsrc.reshape(-1), T, rE, rEsrc, FC, dt, time,
                                 ncells, C.byref(st), nthreads)
         return r, FC
+
+
+class tinv_counts:
+    """Counts T inversions and the energy evaluations of their Newton loops
+    in the calls made inside the ``with`` block (one process; the counters
+    are global to the library): ``.calls``, ``.evals``, ``.mean``."""
+
+    def __init__(self, variant: str = "fma"):
+        self.L = lib(variant)
+        self.calls = self.evals = 0
+
+    def __enter__(self):
+        self.L.ppo_tinv_count(1)
+        return self
+
+    def __exit__(self, *exc):
+        c, e = C.c_longlong(), C.c_longlong()
+        self.L.ppo_tinv_stats(C.byref(c), C.byref(e))
+        self.calls, self.evals = c.value, e.value
+        self.L.ppo_tinv_count(0)
+        return False
+
+    @property
+    def mean(self) -> float:
+        return self.evals / max(self.calls, 1)
 
 
 def bdf_linear(A, y, dt, b=None, **kw):

@@ -232,6 +232,21 @@ void ppo_ubms_cvbs(const ppo_mech* m, double T, const double* Y, double* e, doub
 /* GET_T_GIVEN_HY / _EY (mechanism.H:2789-2890): Newton on T with |dT| clamped to
  * 100 K, stop when |dT| < 1e-6 (without applying it), 200 iterations, linear
  * extrapolation outside [90, 4000] K. */
+/* Opt-in count of T inversions and of the energy evaluations of their Newton
+ * loops (the bound evaluations excluded), so that the static operation count
+ * of the RHS can use the iterations a workload really takes; off by default,
+ * so the timed CPU runs pay one untaken branch. */
+static int tinv_count_on = 0;
+static long long tinv_calls = 0, tinv_evals = 0;
+void ppo_tinv_count(int on) {
+  tinv_count_on = on;
+  tinv_calls = tinv_evals = 0;
+}
+void ppo_tinv_stats(long long* calls, long long* evals) {
+  *calls = tinv_calls;
+  *evals = tinv_evals;
+}
+
 static double T_given(const ppo_mech* m, int use_h, double hv, const double* Y,
                       double t, int* ierr) {
   const int maxiter = 200;
@@ -257,13 +272,21 @@ static double T_given(const ppo_mech* m, int use_h, double hv, const double* Y,
   }
   t1 = t;
   if (t1 < tmin || t1 > tmax) t1 = tmin + (tmax - tmin) / (hmax - hmin) * (hv - hmin);
+  int nev = 0;
   for (int i = 0; i < maxiter; ++i) {
     if (use_h) ppo_hbms_cpbs(m, t1, Y, &h1, &c); else ppo_ubms_cvbs(m, t1, Y, &h1, &c);
+    ++nev;
     dt = (hv - h1) / c;
     if (dt > 100.) dt = 100.;
     else if (dt < -100.) dt = -100.;
     else if (fabs(dt) < tol) break;
     t1 += dt;
+  }
+  if (tinv_count_on) {
+#pragma omp atomic
+    tinv_calls += 1;
+#pragma omp atomic
+    tinv_evals += nev;
   }
   if (ierr) *ierr = 0;
   return t1;

@@ -68,6 +68,10 @@ void ppo_ubms_cvbs(const ppo_mech* m, double T, const double* Y, double* e, doub
 /* GET_T_GIVEN_HY / GET_T_GIVEN_EY (mechanism.H:2841-2890 / 2789-2838) */
 double ppo_T_given_hy(const ppo_mech* m, double h, const double* Y, double Tguess, int* ierr);
 double ppo_T_given_ey(const ppo_mech* m, double e, const double* Y, double Tguess, int* ierr);
+/* opt-in counters of the above: inversions inside [90, 4000] K and the energy
+ * evaluations of their Newton loops; ppo_tinv_count resets them */
+void ppo_tinv_count(int on);
+void ppo_tinv_stats(long long* calls, long long* evals);
 
 /* --- kinetics in SI-mol units (productionRate mechanism.H:4421, aJacobian :10307) --- */
 void ppo_production_rate(const ppo_mech* m, const double* sc, double T, double* wdot);
