@@ -264,23 +264,54 @@ public:
       amrex::Gpu::dtoh_memcpy(&ep, eosparm, sizeof(ep));
       pele::physics::BlackBoxFunctionData bb;
       amrex::Gpu::dtoh_memcpy(&bb, ep.manf_data, sizeof(bb));
-      if (bb.bbmodel != pele::physics::BlackBoxModel::TABLE) {
-        amrex::Abort("ReactorHIP: the Manifold EOS needs manifold.model = Table");
-      }
-      pele::physics::TabulatedFunctionData td;
-      amrex::Gpu::dtoh_memcpy(&td, ep.manf_data, sizeof(td));
       pp_chem_manifold m{};
-      m.ndim = td.Ndim;
-      m.dim_lengths = td.dimLengths;
-      m.grids = td.grids;
-      m.nvar = td.Nvar;
-      m.values = td.values;
+      pele::physics::TabulatedFunctionData td;
+      // NNModel::pack (NeuralNetModelDef.H:283-305) of a CMLM net, in the
+      // pinned host buffers of NeuralNetFunctionData: ints {1, inputs, then
+      // per layer Linear {nout, nin}, LeakyReLU {}, BatchNorm1d {n}}, reals
+      // the layers' parameters one after the other
+      int nn_kind[7], nn_nout[7];
+      if (bb.bbmodel == pele::physics::BlackBoxModel::NEURAL_NET) {
+        pele::physics::NeuralNetFunctionData nd;
+        amrex::Gpu::dtoh_memcpy(&nd, ep.manf_data, sizeof(nd));
+        const int* id = nd.nnidata;
+        if (!id || !nd.nnrdata || id[0] != 1) {
+          amrex::Abort("ReactorHIP: Only CMLM nets supported for now without owning data");
+        }
+        id += 2;
+        const int order[7] = {0, 1, 2, 0, 1, 2, 0};  // Linear, LeakyReLU, BatchNorm1d, ...
+        int width = nd.Ndim;
+        for (int l = 0; l < 7; ++l) {
+          nn_kind[l] = order[l];
+          if (order[l] == 0) {
+            width = id[0];
+            id += 2;
+          } else if (order[l] == 2) {
+            id += 1;
+          }
+          nn_nout[l] = width;
+        }
+        m.model = 1;
+        m.nn_layers = 7;
+        m.nn_kind = nn_kind;
+        m.nn_nout = nn_nout;
+        m.nn_params = nd.nnrdata;
+        m.ndim = nd.Ndim;
+        m.nvar = nd.Nvar;
+      } else {
+        amrex::Gpu::dtoh_memcpy(&td, ep.manf_data, sizeof(td));
+        m.ndim = td.Ndim;
+        m.dim_lengths = td.dimLengths;
+        m.grids = td.grids;
+        m.nvar = td.Nvar;
+        m.values = td.values;
+      }
       m.idx_density = ep.idx_density;
       m.density_lookup = static_cast<int>(ep.dens_lookup);
       m.idx_T = ep.idx_T;
       m.compute_temperature = ep.compute_temperature ? 1 : 0;
       for (int k = 0; k < 5; ++k) {
-        m.idx_src[k] = k < td.Ndim ? ep.idx_Wdot[k] : -1;
+        m.idx_src[k] = k < m.ndim ? ep.idx_Wdot[k] : -1;
       }
       m.nominal_pressure = ep.Pnom_cgs;
       if (pp_chem_set_manifold(m_h, &m) != 0) {

@@ -175,6 +175,19 @@ typedef struct {
   int compute_temperature; /* manifold.compute_temperature: 1: T from the table, 0: T = 1 */
   int idx_src[5];          /* source of each manifold parameter (SRC_<name>), -1: none (zero) */
   double nominal_pressure; /* manifold.nominal_pressure_cgs */
+  /* model 1: a network (manifold.model = NeuralNet, NeuralNetHomerolled.H,
+   * NeuralNetModelDef.H, NeuralNetLayerDef.H) in place of the table: ndim
+   * inputs, nvar outputs, dim_lengths / grids / values unused.  Layer l of
+   * nn_layers (at most 16) is nn_kind[l] = 0 Linear (nn_nout[l] outputs;
+   * parameters W[nout][nin] then b[nout]), 1 LeakyReLU (the negative slope)
+   * or 2 BatchNorm1d (scale[n] then shift[n]), each layer narrower than 100;
+   * nn_params holds the layers' parameters one after the other.  model 0
+   * (the zero default) is the table. */
+  int model;
+  int nn_layers;
+  const int* nn_kind;
+  const int* nn_nout;
+  const double* nn_params;
 } pp_chem_manifold;
 
 /* ReactorRK64::set_eos_parm for the Manifold EOS: the table every lookup of

@@ -1643,7 +1643,48 @@ static void mf_point(const ppo_manifold* t, const double* x, int* idx, double* w
   *nper = st;
 }
 
+/* NNModel::operator() with the layers of NeuralNetLayerDef.H: Linear
+ * (outputs zeroed, then in_i W[j][i] added in i order, then the bias),
+ * LeakyReLU clamp(v, 0, inf) + slope clamp(v, -inf, 0), BatchNorm1d
+ * v scale + shift; the two buffers alternate (100 values each). */
+static const double* mf_nn(const ppo_manifold* t, const double* x, double* a, double* b) {
+  const double* in = x;
+  double* out = a;
+  const double* p = t->nn_params;
+  int nin = t->ndim;
+  for (int l = 0; l < t->nn_layers; ++l) {
+    int nout = nin;
+    if (t->nn_kind[l] == 0) {
+      nout = t->nn_nout[l];
+      for (int j = 0; j < nout; ++j) out[j] = 0.0;
+      for (int j = 0; j < nout; ++j) {
+        for (int i = 0; i < nin; ++i) out[j] += in[i] * p[nin * j + i];
+        out[j] += p[nin * nout + j];
+      }
+      p += nin * nout + nout;
+    } else if (t->nn_kind[l] == 1) {
+      for (int i = 0; i < nin; ++i) {
+        double lo = in[i] < 0.0 ? 0.0 : in[i];
+        double hi = in[i] > 0.0 ? 0.0 : in[i];
+        out[i] = lo + p[0] * hi;
+      }
+      p += 1;
+    } else {
+      for (int i = 0; i < nin; ++i) out[i] = in[i] * p[i] + p[nin + i];
+      p += 2 * nin;
+    }
+    nin = nout;
+    in = out;
+    out = out == a ? b : a;
+  }
+  return in;
+}
+
 double ppo_manifold_value(const ppo_manifold* t, int ivar, const double* x) {
+  if (t->model == 1) {
+    double a[100], b[100];
+    return mf_nn(t, x, a, b)[ivar];
+  }
   int idx[5], stride[5];
   double w[5];
   size_t nper;
@@ -1672,6 +1713,14 @@ static void manifold_rhs(const sys_t* s, const double* y, double* f) {
   const int D = t->ndim;
   double rho = y[D], rinv = 1.0 / rho, Y[5];
   for (int k = 0; k < D; ++k) Y[k] = y[k] * rinv;
+  if (t->model == 1) {
+    double a[100], b[100];
+    const double* o = mf_nn(t, Y, a, b);
+    for (int k = 0; k < D; ++k) f[k] = (t->idx_src[k] >= 0 ? o[t->idx_src[k]] : 0.0) + s->rYsrc[k];
+    f[D] = s->rYsrc[D];
+    f[D + 1] = s->rhoesrc * rinv;
+    return;
+  }
   int idx[5], stride[5];
   double w[5];
   size_t nper;

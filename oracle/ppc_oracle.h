@@ -157,6 +157,13 @@ typedef struct {
   const double* values; /* nvar x prod(dim_len), first dimension fastest */
   int idx_density, dens_lookup, idx_T, compute_T;
   int idx_src[5];       /* -1: no source */
+  /* model 1: a network in place of the table (NeuralNetModelDef.H:227-249):
+   * nn_layers layers of kind 0 Linear (W[nout][nin], b[nout]), 1 LeakyReLU
+   * (slope), 2 BatchNorm1d (scale[n], shift[n]), parameters consecutive */
+  int model, nn_layers;
+  const int* nn_kind;
+  const int* nn_nout;
+  const double* nn_params;
 } ppo_manifold;
 
 /* TabulatedFunction::get_value (Table.H:318-330): variable ivar at x[ndim]. */

@@ -226,7 +226,7 @@ int fail(pp_chem* h, int code, const std::string& msg) {
 // The Manifold EOS has nothing to evaluate before its table is set
 // (Manifold.H:333-338 refuses to construct without it).
 int manifold_unset(pp_chem* h) {
-  return h->manifold && !h->P.man.values
+  return h->manifold && !h->P.man.values && !h->P.man.nn
              ? fail(h, PP_CHEM_EINVAL, "Manifold EOS must be initialized with table data (pp_chem_set_manifold)")
              : 0;
 }
@@ -725,28 +725,66 @@ int pp_chem_set_typ_vals(pp_chem* h, const double* typ_vals, int n) {
 int pp_chem_set_manifold(pp_chem* h, const pp_chem_manifold* m) {
   if (!h) return PP_CHEM_EINVAL;
   if (!h->manifold) return fail(h, PP_CHEM_EINVAL, "manifold parameters need the Manifold EOS (mechanism \"Manifold\")");
-  if (!m || !m->dim_lengths || !m->grids || !m->values) return fail(h, PP_CHEM_EINVAL, "null argument");
+  if (!m) return fail(h, PP_CHEM_EINVAL, "null argument");
+  if (m->model != 0 && m->model != 1) return fail(h, PP_CHEM_EINVAL, "manifold model must be 0 (table) or 1 (network)");
+  const bool net = m->model == 1;
+  if (net ? (!m->nn_kind || !m->nn_nout || !m->nn_params) : (!m->dim_lengths || !m->grids || !m->values))
+    return fail(h, PP_CHEM_EINVAL, "null argument");
   if (m->ndim != h->cfg.manifold_dim)
-    return fail(h, PP_CHEM_EINVAL, "table dimension differs from manifold_dim (NUM_SPECIES - 1)");
+    return fail(h, PP_CHEM_EINVAL,
+                net ? "network inputs differ from manifold_dim (NUM_SPECIES - 1)"
+                    : "table dimension differs from manifold_dim (NUM_SPECIES - 1)");
   if (m->nvar < 1) return fail(h, PP_CHEM_EINVAL, "no tabulated variables");
   ppc::ManifoldDev d{};
   d.ndim = m->ndim;
   d.nvar = m->nvar;
-  long long ngrid = 0, stride = 1;
-  for (int k = 0; k < m->ndim; ++k) {
-    const int n = m->dim_lengths[k];
-    if (n < 2) return fail(h, PP_CHEM_EINVAL, "a table dimension has fewer than 2 points");
-    for (int i = 0; i + 1 < n; ++i)
-      if (!(m->grids[ngrid + i + 1] > m->grids[ngrid + i]))
-        return fail(h, PP_CHEM_EINVAL, "table grids must be strictly ascending");
-    d.dim_len[k] = n;
-    d.dim_off[k] = (int)ngrid;
-    d.dim_stride[k] = (int)stride;
-    ngrid += n;
-    stride *= n;
-    if (stride > 0x7fffffffLL / 8) return fail(h, PP_CHEM_EINVAL, "table too large");
+  d.model = m->model;
+  long long ngrid = 0, stride = 1, nparam = 0;
+  if (net) {
+    if (m->nn_layers < 1 || m->nn_layers > ppc::MAN_NN_MAXL)
+      return fail(h, PP_CHEM_EINVAL, "a network has 1 to 16 layers");
+    int width = m->ndim;
+    for (int l = 0; l < m->nn_layers; ++l) {
+      const int kind = m->nn_kind[l];
+      int nout = width;
+      long long np;
+      if (kind == 0) {
+        nout = m->nn_nout[l];
+        np = (long long)width * nout + nout;
+      } else if (kind == 1) {
+        np = 1;
+      } else if (kind == 2) {
+        np = 2LL * width;
+      } else {
+        return fail(h, PP_CHEM_EINVAL, "network layer kind must be 0 (Linear), 1 (LeakyReLU) or 2 (BatchNorm1d)");
+      }
+      if (nout < 1 || nout >= ppc::MAN_NN_MAXW)
+        return fail(h, PP_CHEM_EINVAL, "network layers must have 1 to 99 outputs (MAXD_NETWORK)");
+      d.nn_kind[l] = kind;
+      d.nn_nin[l] = width;
+      d.nn_nout[l] = nout;
+      d.nn_off[l] = (int)nparam;
+      nparam += np;
+      width = nout;
+    }
+    if (width != m->nvar) return fail(h, PP_CHEM_EINVAL, "the network's last layer must give nvar outputs");
+    d.nn_layers = m->nn_layers;
+  } else {
+    for (int k = 0; k < m->ndim; ++k) {
+      const int n = m->dim_lengths[k];
+      if (n < 2) return fail(h, PP_CHEM_EINVAL, "a table dimension has fewer than 2 points");
+      for (int i = 0; i + 1 < n; ++i)
+        if (!(m->grids[ngrid + i + 1] > m->grids[ngrid + i]))
+          return fail(h, PP_CHEM_EINVAL, "table grids must be strictly ascending");
+      d.dim_len[k] = n;
+      d.dim_off[k] = (int)ngrid;
+      d.dim_stride[k] = (int)stride;
+      ngrid += n;
+      stride *= n;
+      if (stride > 0x7fffffffLL / 8) return fail(h, PP_CHEM_EINVAL, "table too large");
+    }
+    d.var_stride = (int)stride;
   }
-  d.var_stride = (int)stride;
   const auto var_ok = [&](int v, bool opt) { return (opt && v == -1) || (v >= 0 && v < m->nvar); };
   if (!var_ok(m->idx_density, false) || !var_ok(m->idx_T, false))
     return fail(h, PP_CHEM_EINVAL, "density or temperature variable index out of range");
@@ -759,16 +797,21 @@ int pp_chem_set_manifold(pp_chem* h, const pp_chem_manifold* m) {
   d.compute_T = m->compute_temperature ? 1 : 0;
   for (int k = 0; k < ppc::MAN_MAXD; ++k) d.idx_src[k] = k < m->ndim ? m->idx_src[k] : -1;
   d.pnom = m->nominal_pressure;
-  const size_t nval = (size_t)m->nvar * (size_t)stride;
+  const size_t nval = net ? (size_t)nparam : (size_t)m->nvar * (size_t)stride;
   HIPCHK(h, hipSetDevice(h->device));
   if (h->d_man) HIPCHK(h, hipFree(h->d_man));
   h->d_man = nullptr;
   h->P.man = ppc::ManifoldDev{};
   HIPCHK(h, hipMalloc(&h->d_man, sizeof(double) * ((size_t)ngrid + nval)));
-  HIPCHK(h, hipMemcpy(h->d_man, m->grids, sizeof(double) * (size_t)ngrid, hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(h->d_man + ngrid, m->values, sizeof(double) * nval, hipMemcpyHostToDevice));
-  d.grids = h->d_man;
-  d.values = h->d_man + ngrid;
+  if (net) {
+    HIPCHK(h, hipMemcpy(h->d_man, m->nn_params, sizeof(double) * nval, hipMemcpyHostToDevice));
+    d.nn = h->d_man;
+  } else {
+    HIPCHK(h, hipMemcpy(h->d_man, m->grids, sizeof(double) * (size_t)ngrid, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_man + ngrid, m->values, sizeof(double) * nval, hipMemcpyHostToDevice));
+    d.grids = h->d_man;
+    d.values = h->d_man + ngrid;
+  }
   h->P.man = d;
   drop_graph(h);
   return PP_CHEM_OK;

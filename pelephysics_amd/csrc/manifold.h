@@ -2,7 +2,8 @@
 // parameters plus the density (Source/Eos/Manifold.H, Mechanisms/Null:
 // NUM_SPECIES = D + 1, the last "species" is rho), every EOS query and the
 // parameter source terms are multilinear interpolations in a table
-// (Source/Utility/BlackBoxFunction/Table.H).
+// (Source/Utility/BlackBoxFunction/Table.H) or outputs of a small dense network
+// (manifold.model = NeuralNet, NeuralNetHomerolled.H), evaluated per lane.
 //
 //  * lookup: the left grid index of each coordinate (bisection, clamped to
 //    [0, len - 2]), alpha = (x_{i+1} - x) / (x_{i+1} - x_i) clamped to [0, 1],
@@ -70,11 +71,60 @@ struct ManPoint {
   }
 };
 
+// NNModel::operator() (NeuralNetModelDef.H:227-249, the layers of
+// NeuralNetLayerDef.H): Linear out_j = sum_i in_i W[j][i] (summed in i order)
+// + b_j; LeakyReLU max-clamp plus slope x min-clamp (a NaN passes through);
+// BatchNorm1d in_i scale_i + shift_i.  a and b hold MAN_NN_MAXW values each;
+// returns the buffer with the outputs.
+PPC_DI const double* man_nn_forward(const ManifoldDev& m, const double* x, double* a, double* b) {
+  const double* in = x;
+  double* out = a;
+  for (int l = 0; l < m.nn_layers; ++l) {
+    const double* p = m.nn + m.nn_off[l];
+    const int nin = m.nn_nin[l], nout = m.nn_nout[l];
+    if (m.nn_kind[l] == 0) {
+      for (int j = 0; j < nout; ++j) {
+        double s = 0.0;
+        for (int i = 0; i < nin; ++i) s += in[i] * p[nin * j + i];
+        out[j] = s + p[nin * nout + j];
+      }
+    } else if (m.nn_kind[l] == 1) {
+      const double slope = p[0];
+      for (int i = 0; i < nin; ++i) {
+        const double v = in[i];
+        out[i] = (v < 0.0 ? 0.0 : v) + slope * (v > 0.0 ? 0.0 : v);
+      }
+    } else {
+      for (int i = 0; i < nin; ++i) out[i] = in[i] * p[i] + p[nin + i];
+    }
+    in = out;
+    out = out == a ? b : a;
+  }
+  return in;
+}
+
+// BlackBoxFunction::get_values (BlackBoxFunction.H:63-67): the variables
+// ivar[0..nreq) at the manifold point Y (a negative index gives 0), from the
+// table or from one evaluation of the network.
+template <int D>
+PPC_DI void man_values(const ManifoldDev& m, const double* Y, int nreq, const int* ivar, double* out) {
+  if (m.model == 1) {
+    double a[MAN_NN_MAXW], b[MAN_NN_MAXW];
+    const double* o = man_nn_forward(m, Y, a, b);
+    for (int i = 0; i < nreq; ++i) out[i] = ivar[i] >= 0 ? o[ivar[i]] : 0.0;
+  } else {
+    const ManPoint<D> pt(m, Y);
+    for (int i = 0; i < nreq; ++i) out[i] = ivar[i] >= 0 ? pt.value(m, ivar[i]) : 0.0;
+  }
+}
+
 // Manifold::RHY2T / REY2T (Manifold.H:176-210).
 template <int D>
 PPC_DI double man_temperature(const ManifoldDev& m, const double* Y) {
   if (!m.compute_T) return 1.0;
-  return ManPoint<D>(m, Y).value(m, m.idx_T);
+  double T;
+  man_values<D>(m, Y, 1, &m.idx_T, &T);
+  return T;
 }
 
 // fKernelSpec with the Manifold EOS.
@@ -86,9 +136,10 @@ PPC_DI void man_rhs(const ManifoldDev& m, const double* __restrict__ y, double* 
   double Y[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) Y[d] = y[d] * rinv;
-  const ManPoint<D> pt(m, Y);
+  double wdot[D];
+  man_values<D>(m, Y, D, m.idx_src, wdot);
 #pragma unroll
-  for (int d = 0; d < D; ++d) ydot[d] = (m.idx_src[d] >= 0 ? pt.value(m, m.idx_src[d]) : 0.0) + rYsrc[d];
+  for (int d = 0; d < D; ++d) ydot[d] = wdot[d] + rYsrc[d];
   ydot[D] = 0.0 + rYsrc[D];
   ydot[D + 1] = rhoesrc_ext * (rinv / 1.0);
 }

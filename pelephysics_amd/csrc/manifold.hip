@@ -10,7 +10,7 @@ namespace {
 template <int D>
 hipError_t man_1d(int mode, const ReactParamsDev& P, double* y, const double* ysrc, double* rX, const double* rXsrc,
                   double dt, int ncells, CellStats st, int block, hipStream_t s) {
-  if (mode != MODE_RK64 || P.man.ndim != D || !P.man.values || block < 1) return hipErrorInvalidValue;
+  if (mode != MODE_RK64 || P.man.ndim != D || (!P.man.values && !P.man.nn) || block < 1) return hipErrorInvalidValue;
   const int grid = (ncells + block - 1) / block;
   hipLaunchKernelGGL(man_react_1d_kernel<D>, dim3(grid), dim3(block), 0, s, P, y, ysrc, rX, rXsrc, dt, ncells, st);
   return hipGetLastError();
@@ -19,7 +19,7 @@ hipError_t man_1d(int mode, const ReactParamsDev& P, double* y, const double* ys
 template <int D>
 hipError_t man_box(int mode, const ReactParamsDev& P, const BoxArgs& b, double dt, int ncells, CellStats st,
                    int block, hipStream_t s) {
-  if (mode != MODE_RK64 || P.man.ndim != D || !P.man.values || block < 1) return hipErrorInvalidValue;
+  if (mode != MODE_RK64 || P.man.ndim != D || (!P.man.values && !P.man.nn) || block < 1) return hipErrorInvalidValue;
   const int grid = (ncells + block - 1) / block;
   hipLaunchKernelGGL(man_react_box_kernel<D>, dim3(grid), dim3(block), 0, s, P, b, dt, ncells, st);
   return hipGetLastError();

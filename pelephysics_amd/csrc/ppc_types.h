@@ -33,6 +33,8 @@ struct AdaptParamsDev {  // adaptive BDF tolerances
 // parameters), nvar tabulated variables on the tensor grid, first dimension
 // fastest.
 constexpr int MAN_MAXD = 5;  // MAXD_TABLE (Table.H:4)
+constexpr int MAN_NN_MAXL = 16;   // network layers
+constexpr int MAN_NN_MAXW = 100;  // network width bound (MAXD_NETWORK)
 struct ManifoldDev {
   int ndim, nvar;
   int dim_len[MAN_MAXD];     // grid points per dimension (>= 2)
@@ -47,6 +49,12 @@ struct ManifoldDev {
   int compute_T;             // 1: T from the table; 0: T = 1
   int idx_src[MAN_MAXD];     // source term of each manifold parameter, -1: none (zero)
   double pnom;               // nominal pressure, dyn/cm^2
+  // model 1: a network in place of the table (layers in order; kinds 0
+  // Linear, 1 LeakyReLU, 2 BatchNorm1d; their parameters at nn + nn_off[l])
+  int model;
+  int nn_layers;
+  int nn_kind[MAN_NN_MAXL], nn_nin[MAN_NN_MAXL], nn_nout[MAN_NN_MAXL], nn_off[MAN_NN_MAXL];
+  const double* nn;
 };
 
 struct ErkParamsDev {  // ReactorArkode: the table of ode.rk_method and ode.rk_controller

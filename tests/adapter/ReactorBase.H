@@ -96,6 +96,11 @@ struct BlackBoxFunctionData {
   char* dimnames;
   char* varnames;
 };
+struct NeuralNetFunctionData : BlackBoxFunctionData {  // NeuralNetHomerolled.H:14-19
+  double* nnrdata;
+  int* nnidata;
+  char nn_filename[len_str + 1];
+};
 struct TabulatedFunctionData : BlackBoxFunctionData {
   int varSpacing;
   int* dimLengths;
